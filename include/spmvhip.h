@@ -1,0 +1,169 @@
+/*
+ * spmvhip.h -- C ABI of the gfx950 (MI355X) FP64 SpMV layer.
+ *
+ * The reference's plugin contract is a C++ struct per format
+ * (src/spmv_kernels/spmv_kernel.h:8-29):
+ *
+ *   struct Matrix_Format { char *format_name; long m, n, nnz;
+ *       double mem_footprint, csr_mem_footprint;
+ *       virtual void spmv(ValueType *x, ValueType *y) = 0;
+ *       virtual void statistics_start() = 0;
+ *       virtual int statistics_print_data(char *buf, long buf_n) = 0; };
+ *   struct Matrix_Format *csr_to_format(INT_T *row_ptr, INT_T *col_ind,
+ *       ValueTypeReference *values, long m, long n, long nnz,
+ *       long symmetric, long symmetry_expanded);
+ *   int statistics_print_labels(char *buf, long buf_n);
+ *
+ * Each entry point below is the plain-C image of one of those members, with
+ * an explicit handle instead of `this`, the format chosen by a parameter block
+ * instead of by which executable was linked, and a status code instead of
+ * exit(). The C++ Matrix_Format subclasses (spmv-dare_amd/host/) are thin
+ * wrappers over these calls.
+ *
+ * Types: INT_T = int32_t, ValueType = ValueTypeReference = double
+ * (src/make.sh:127,134,145).
+ *
+ * Errors: 0 = success, negative SPMVHIP_E* otherwise; spmvhip_last_error()
+ * returns the message of the calling thread's last failure.
+ *
+ * Threading: handles are independent (no file-scope state per format, unlike
+ * the reference's thread_i_s / tds globals); one handle must not be used from
+ * two host threads at once. Each handle owns one HIP stream of default
+ * priority; all work of a handle is ordered on it.
+ */
+#ifndef SPMVHIP_H
+#define SPMVHIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+	SPMVHIP_OK = 0,
+	SPMVHIP_EINVAL = -1,      /* invalid argument or unsupported parameter */
+	SPMVHIP_ENOMEM = -2,      /* device allocation failed */
+	SPMVHIP_EHIP = -3,        /* HIP runtime error */
+	SPMVHIP_ENODEV = -4,      /* no usable gfx950 device */
+	SPMVHIP_EFORMAT = -5,     /* input is not a valid CSR matrix */
+	SPMVHIP_ESYMMETRIC = -6,  /* symmetric && !symmetry_expanded (reference: error()) */
+};
+
+/* Drop-in formats (the executables spmv_<fmt>_d.exe of config.sh:183-196). */
+enum spmvhip_kind {
+	SPMVHIP_CSR_NAIVE = 0,  /* src/spmv_kernels/csr.cpp: one thread per row, sequential, no FMA */
+	SPMVHIP_CSR_VEC = 1,    /* src/spmv_kernels/csr_vec.cpp: row-length-binned wave64 CSR */
+	SPMVHIP_SELL = 2,       /* src/spmv_kernels/sell.cpp: slices of C rows, no sort */
+	SPMVHIP_SELL_C_S = 3,   /* src/spmv_kernels/sell_c_s.cpp: SELL-C-sigma */
+	SPMVHIP_DIV_RF = 4,     /* src/spmv_kernels/DIV/div.cpp: DIV-RF packet stream */
+	SPMVHIP_NUM_KINDS = 5
+};
+
+/* Tuning / format parameters. Zero-initialise and set `kind`; zero fields
+ * take the defaults noted. */
+typedef struct spmvhip_params {
+	int kind;             /* enum spmvhip_kind */
+	int C;                /* SELL slice height; must be 64 (one wavefront) -- default 64 */
+	long sigma;           /* SELL-C-sigma sort window, 1..16384 -- default 4096 */
+	int lanes_per_row;    /* CSR_VEC: 0 = auto; else lanes of the LDS reduction per row (1..64) */
+	int block_nnz;        /* CSR_VEC: nnz per row block staged in LDS (64..2048, default 512) */
+	int split_nnz;        /* CSR_VEC: rows longer than this are split across waves (default 8192) */
+	int xcd_remap;        /* 1 = give each XCD a contiguous range of work (default 0) */
+	int x_window;         /* SELL*: 1 = stage the slice's x window in LDS when it fits */
+	int unroll;           /* SELL*: inner unroll (1, 2, 4, 8; default 4) */
+	int div_threads;      /* DIV_RF: encoder partitions T (default 1) */
+	int div_lanes;        /* DIV_RF: lanes L of the value streams (default 64) */
+	long div_packet_vals; /* DIV_RF: P, max values per packet (default 16384) */
+	int reserved[8];
+} spmvhip_params;
+
+typedef struct spmvhip_format spmvhip_format;   /* opaque */
+
+typedef struct spmvhip_info {
+	const char *format_name; /* Matrix_Format::format_name */
+	long m, n, nnz;
+	double mem_footprint;     /* bytes of the device representation */
+	double csr_mem_footprint; /* nnz*(8+4) + (m+1)*4, as spmv_kernel.h:21-24 */
+	double conv_ms;           /* device time of csr_to_format (after H2D) */
+	long padded_nnz;          /* SELL*: stored entries incl. padding; else nnz */
+	long num_work_items;      /* CSR_VEC: row blocks; SELL*: slices; DIV: packets */
+	long num_split_rows;      /* CSR_VEC: rows split across waves */
+} spmvhip_info;
+
+/* ---- device / memory helpers (no PyTorch needed by callers) ---- */
+int spmvhip_init(int device);
+int spmvhip_device_name(char *buf, long buf_n);
+int spmvhip_malloc(void **dptr, size_t bytes);
+int spmvhip_free(void *dptr);
+int spmvhip_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int spmvhip_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int spmvhip_memset(void *dptr, int value, size_t bytes);
+int spmvhip_synchronize(void);
+const char *spmvhip_last_error(void);
+
+/* ---- the plugin contract ---- */
+
+/* csr_to_format (spmv_kernel.h:28): copies the host CSR to the device and
+ * converts it there. The handle keeps no host pointer. */
+int spmvhip_csr_to_format(const int32_t *row_ptr, const int32_t *col_ind,
+		const double *values, long m, long n, long nnz, long symmetric,
+		long symmetry_expanded, const spmvhip_params *params,
+		spmvhip_format **out);
+
+/* Same, from CSR arrays already resident on the device (not retained). */
+int spmvhip_csr_to_format_device(const int32_t *d_row_ptr, const int32_t *d_col_ind,
+		const double *d_values, long m, long n, long nnz,
+		const spmvhip_params *params, spmvhip_format **out);
+
+/* Matrix_Format::spmv (spmv_kernel.h:17) with HOST x and y, as the reference
+ * driver passes them: x (n doubles) is uploaded, y (m doubles) written back;
+ * every y[i], i < m, is written (empty rows give 0) and nothing past m.
+ * If x and y are the pointers bound with spmvhip_bind_host_xy, no copy is
+ * made: the kernel runs on the bound device mirrors and y stays on the
+ * device until spmvhip_sync_host_y. */
+int spmvhip_spmv(spmvhip_format *f, const double *x, double *y);
+
+/* y = A x with x, y in device memory; enqueued on the handle's stream
+ * (stream == NULL) or on `stream` (a hipStream_t). Asynchronous. */
+int spmvhip_spmv_device(spmvhip_format *f, const double *d_x, double *d_y, void *stream);
+
+/* Device-resident mode of the driver (GPU_KERNEL=1): bind host x/y once. */
+int spmvhip_bind_host_xy(spmvhip_format *f, const double *x, double *y);
+int spmvhip_sync_host_y(spmvhip_format *f);
+
+/* Matrix_Format::statistics_start / statistics_print_data and
+ * statistics_print_labels (spmv_kernel.h:18-19,29): ','-prefixed CSV
+ * columns gbps_alg,pct_peak,kernel_ms,conv_ms,variant. */
+int spmvhip_statistics_start(spmvhip_format *f);
+int spmvhip_statistics_print_data(spmvhip_format *f, char *buf, long buf_n);
+int spmvhip_statistics_print_labels(char *buf, long buf_n);
+
+int spmvhip_get_info(const spmvhip_format *f, spmvhip_info *info);
+
+/* Time `iters` back-to-back spmv_device calls after `warmup` untimed ones,
+ * with HIP events on the handle's stream; ms[i] receives each iteration's
+ * time (ms may be NULL, then only *total_ms). */
+int spmvhip_time_spmv(spmvhip_format *f, const double *d_x, double *d_y,
+		int warmup, int iters, float *ms, float *total_ms);
+
+/* Converted arrays, copied to host for parity checks.
+ * which: 0 row_order (int32, m)      [SELL_C_S]
+ *        1 slice_widths (int32, nslices)   [SELL, SELL_C_S]
+ *        2 slice_ptr (int64, nslices+1)    [SELL, SELL_C_S]
+ *        3 cols (int32, padded_nnz)        [SELL, SELL_C_S]
+ *        4 vals (float64, padded_nnz)      [SELL, SELL_C_S]
+ *        5 row blocks (int32, work_items+1) [CSR_VEC]
+ *        6 DIV decoded rows (int32, nnz), 7 DIV decoded cols (int32, nnz),
+ *        8 DIV decoded values (float64, nnz) -- device decode, packet order [DIV_RF]
+ * *bytes_inout: capacity in, bytes written out (or needed, with buf NULL). */
+int spmvhip_export(const spmvhip_format *f, int which, void *buf, size_t *bytes_inout);
+
+void spmvhip_destroy(spmvhip_format *f);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPMVHIP_H */

@@ -1,0 +1,324 @@
+/*
+ * CPU restatement of the reference SpMV path -- TEST INFRASTRUCTURE ONLY
+ * (the checker; never linked into the product). Declarations, pinning notes
+ * and reference citations: oracle.h. Compiled with -ffp-contract=off, so
+ * every product and sum below is rounded exactly where it is written.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+#include "oracle.h"
+
+/* ------------------------------------------------------------ csr_naive */
+
+void oracle_csr_naive(const int32_t *rp, const int32_t *col, const double *val,
+		const double *x, double *y, long m)
+{
+	for (long i = 0; i < m; i++) {
+		double sum = 0;
+		for (long j = rp[i]; j < rp[i + 1]; j++) {
+			double prod = val[j] * x[col[j]];
+			sum = sum + prod;
+		}
+		y[i] = sum;
+	}
+}
+
+/* ------------------------------------------------------------ double-double */
+
+static inline void two_sum(double a, double b, double *s, double *e)
+{
+	double t = a + b;
+	double bb = t - a;
+	*e = (a - (t - bb)) + (b - bb);
+	*s = t;
+}
+
+void oracle_csr_dd(const int32_t *rp, const int32_t *col, const double *val,
+		const double *x, double *y_hi, double *y_lo, long m)
+{
+#pragma omp parallel for schedule(dynamic, 1024)
+	for (long i = 0; i < m; i++) {
+		double hi = 0, lo = 0;
+		for (long j = rp[i]; j < rp[i + 1]; j++) {
+			double a = val[j], b = x[col[j]];
+			double p = a * b;
+			double pe = fma(a, b, -p); /* exact product error */
+			double s, e;
+			two_sum(hi, p, &s, &e);
+			e = e + (lo + pe);
+			hi = s + e;             /* fast two-sum renormalisation */
+			lo = e - (hi - s);
+		}
+		y_hi[i] = hi;
+		y_lo[i] = lo;
+	}
+}
+
+/* ------------------------------------------------------------ partitioners */
+
+void oracle_partition_iterations(long W, long w, long start, long end, long *s, long *e)
+{
+	long len = end - start;
+	if (1 > len) { /* |incr| >= len: worker 0 takes everything */
+		*s = (w == 0) ? start : end;
+		*e = end;
+		return;
+	}
+	long per = len / W, rem = len % W;
+	if (rem != 0 && w < rem) {
+		per += 1;
+		rem = 0;
+	}
+	long ls = start + per * w + rem;
+	long le = ls + per;
+	if (w == W - 1)
+		le = end;
+	*s = ls;
+	*e = le;
+}
+
+/* closest index to `target` in the non-decreasing sums[lo..hi]; ties in
+ * distance go to the upper index (macrolib.h:537-575, default cmp/dist). */
+static long closest_index(const int64_t *A, long lo, long hi, int64_t target)
+{
+	long s = lo, e = hi, mid;
+	if (target < A[s])
+		return s;
+	if (target > A[e])
+		return e;
+	for (;;) {
+		mid = (s + e) / 2;
+		if (mid == s || mid == e)
+			break;
+		if (target > A[mid])
+			s = mid;
+		else
+			e = mid;
+	}
+	if (A[s] == target)
+		return s;
+	if (A[e] == target)
+		return e;
+	int64_t ds = target - A[s], de = A[e] - target;
+	return ds < de ? s : e;
+}
+
+void oracle_partition_prefix_sums(long W, long w, const int64_t *sums, long N,
+		int64_t total, long *s, long *e)
+{
+	int64_t target = sums[0] + (total * w) / W;
+	int64_t target_next = sums[0] + (total * (w + 1)) / W;
+	*s = (w == 0) ? 0 : closest_index(sums, 0, N - 1, target);
+	*e = (w == W - 1) ? N : closest_index(sums, 0, N - 1, target_next);
+}
+
+/* ------------------------------------------------------------ OpenMP csr_naive */
+
+int oracle_csr_naive_omp(const int32_t *rp, const int32_t *col, const double *val,
+		const double *x, double *y, long m, int threads)
+{
+	int T = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel num_threads(T)
+	{
+		long s, e;
+		oracle_partition_iterations(omp_get_num_threads(), omp_get_thread_num(), 0, m, &s, &e);
+		long j = rp[s];
+		for (long i = s; i < e; i++) {
+			long je = rp[i + 1];
+			double sum = 0;
+			for (; j < je; j++)
+				sum = sum + val[j] * x[col[j]];
+			y[i] = sum;
+		}
+	}
+	return T;
+}
+
+/* ------------------------------------------------------------ SELL-C-sigma */
+
+void oracle_sellcs_order(const int32_t *rp, long m, long sigma, int64_t *order)
+{
+	/* stable counting sort by descending length inside each window: the
+	 * rows of one length keep their ascending id order */
+	long win = sigma > 0 ? sigma : 1;
+	for (long w0 = 0; w0 < m; w0 += win) {
+		long wl = m - w0 < win ? m - w0 : win;
+		int32_t mx = 0;
+		for (long i = 0; i < wl; i++) {
+			int32_t len = rp[w0 + i + 1] - rp[w0 + i];
+			mx = len > mx ? len : mx;
+		}
+		long *start = calloc((size_t) mx + 2, sizeof(long));
+		for (long i = 0; i < wl; i++)
+			start[rp[w0 + i + 1] - rp[w0 + i]]++;
+		long run = 0;
+		for (long len = mx; len >= 0; len--) { /* longest first */
+			long c = start[len];
+			start[len] = run;
+			run += c;
+		}
+		for (long i = 0; i < wl; i++) {
+			int32_t len = rp[w0 + i + 1] - rp[w0 + i];
+			order[w0 + start[len]++] = w0 + i;
+		}
+		free(start);
+	}
+}
+
+long oracle_sellcs_layout(const int32_t *rp, long m, long C, const int64_t *order,
+		int width_rule, int64_t *widths, int64_t *slice_ptr)
+{
+	long ns = (m + C - 1) / C;
+	int64_t total = 0;
+	for (long s = 0; s < ns; s++) {
+		int64_t w;
+		if (width_rule == 0) {
+			int64_t r = order[s * C];
+			w = rp[r + 1] - rp[r];
+		} else {
+			w = 0;
+			for (long l = 0; l < C && s * C + l < m; l++) {
+				int64_t r = order[s * C + l];
+				int64_t k = rp[r + 1] - rp[r];
+				w = k > w ? k : w;
+			}
+		}
+		widths[s] = w;
+		slice_ptr[s] = total;
+		total += w * C;
+	}
+	slice_ptr[ns] = total;
+	return total;
+}
+
+long oracle_sellcs_fill(const int32_t *rp, const int32_t *col, const double *val, long m,
+		long C, const int64_t *order, const int64_t *widths, const int64_t *slice_ptr,
+		int64_t *cols_out, double *vals_out)
+{
+	long ns = (m + C - 1) / C;
+	long truncated = 0;
+	memset(cols_out, 0, (size_t) slice_ptr[ns] * sizeof(int64_t));
+	memset(vals_out, 0, (size_t) slice_ptr[ns] * sizeof(double));
+	for (long r = 0; r < m; r++) {
+		long s = r / C;
+		int64_t at = slice_ptr[s] + (r % C);
+		int64_t src = order[r];
+		long len = rp[src + 1] - rp[src];
+		long keep = len <= widths[s] ? len : widths[s];
+		truncated += len - keep;
+		for (long k = 0; k < keep; k++) {
+			cols_out[at] = col[rp[src] + k];
+			vals_out[at] = val[rp[src] + k];
+			at += C;
+		}
+	}
+	return truncated;
+}
+
+void oracle_sellcs_spmv(long m, long C, const int64_t *order, const int64_t *widths,
+		const int64_t *slice_ptr, const int64_t *cols, const double *vals,
+		const double *x, double *y)
+{
+	long ns = (m + C - 1) / C;
+	for (long s = 0; s < ns; s++) {
+		for (long l = 0; l < C && s * C + l < m; l++) {
+			double acc = 0;
+			const int64_t base = slice_ptr[s] + l;
+			for (long k = 0; k < widths[s]; k++)
+				acc = fma(vals[base + k * C], x[cols[base + k * C]], acc);
+			y[order[s * C + l]] = acc;
+		}
+	}
+}
+
+/* ------------------------------------------------------------ SELL */
+
+long oracle_sell_layout(const int32_t *rp, long m, long VL, int64_t *cluster_ptr)
+{
+	long nc = (m + VL - 1) / VL;
+	int64_t total = 0;
+	for (long c = 0; c < nc; c++) {
+		int64_t w = 0;
+		for (long i = c * VL; i < (c + 1) * VL && i < m; i++) {
+			int64_t d = rp[i + 1] - rp[i];
+			w = d > w ? d : w;
+		}
+		cluster_ptr[c] = total;
+		total += VL * w;
+	}
+	cluster_ptr[nc] = total;
+	return total;
+}
+
+void oracle_sell_fill(const int32_t *rp, const int32_t *col, const double *val, long m,
+		long VL, long T, const int64_t *cluster_ptr, int64_t *ja, double *a)
+{
+	long nc = (m + VL - 1) / VL;
+	int64_t nnz_ext = cluster_ptr[nc];
+	int64_t *rowmajor_j = malloc((size_t) (nnz_ext > 0 ? nnz_ext : 1) * sizeof(int64_t));
+	double *rowmajor_a = malloc((size_t) (nnz_ext > 0 ? nnz_ext : 1) * sizeof(double));
+	for (long t = 0; t < T; t++) {
+		long c0, c1;
+		oracle_partition_prefix_sums(T, t, cluster_ptr, nc, nnz_ext, &c0, &c1);
+		int64_t colv = 0; /* per-thread carry (sell.cpp:124,165) */
+		for (long c = c0; c < c1; c++) {
+			int64_t w = (cluster_ptr[c + 1] - cluster_ptr[c]) / VL;
+			long i0 = VL * c, i1 = i0 + VL < m ? i0 + VL : m;
+			int64_t jj = cluster_ptr[c];
+			for (long i = i0; i < i1; i++) {
+				long j = rp[i];
+				for (; j < rp[i + 1]; j++, jj++) {
+					rowmajor_a[jj] = val[j];
+					colv = col[j];
+					rowmajor_j[jj] = colv;
+				}
+				for (; j < rp[i] + w; j++, jj++) {
+					rowmajor_a[jj] = 0;
+					rowmajor_j[jj] = colv;
+				}
+			}
+			for (; jj < cluster_ptr[c + 1]; jj++) {
+				rowmajor_a[jj] = 0;
+				rowmajor_j[jj] = colv;
+			}
+			/* transpose VL x w (row-major) into w x VL (sell.cpp:78-92,193-194) */
+			for (int64_t k = 0; k < w; k++)
+				for (long l = 0; l < VL; l++) {
+					ja[cluster_ptr[c] + k * VL + l] = rowmajor_j[cluster_ptr[c] + l * w + k];
+					a[cluster_ptr[c] + k * VL + l] = rowmajor_a[cluster_ptr[c] + l * w + k];
+				}
+		}
+	}
+	free(rowmajor_j);
+	free(rowmajor_a);
+}
+
+void oracle_sell_spmv(long m, long VL, const int64_t *cluster_ptr, const int64_t *ja,
+		const double *a, const double *x, double *y)
+{
+	long nc = (m + VL - 1) / VL;
+	for (long c = 0; c < nc; c++) {
+		for (long l = 0; l < VL && c * VL + l < m; l++) {
+			double acc = 0;
+			for (int64_t jj = cluster_ptr[c] + l; jj < cluster_ptr[c + 1]; jj += VL)
+				acc = fma(a[jj], x[ja[jj]], acc);
+			y[c * VL + l] = acc;
+		}
+	}
+}
+
+/* ------------------------------------------------------------ fixtures */
+
+uint64_t oracle_fnv1a(const void *p, long nbytes)
+{
+	const unsigned char *b = (const unsigned char *) p;
+	uint64_t h = 1469598103934665603ULL;
+	for (long i = 0; i < nbytes; i++) {
+		h ^= b[i];
+		h *= 1099511628211ULL;
+	}
+	return h;
+}

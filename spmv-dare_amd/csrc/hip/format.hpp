@@ -1,0 +1,69 @@
+// Internal state of one converted matrix (the object behind spmvhip_format*).
+#pragma once
+
+#include "common.hpp"
+
+struct spmvhip_format {
+	spmvhip_params p{};
+	int kind = 0;
+	long m = 0, n = 0, nnz = 0;
+	const char *name = "";
+	hipStream_t stream = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+	// CSR arrays (all CSR kinds; DIV keeps them only until encoded)
+	spmvhip::DevBuf row_ptr, col, val;
+
+	// CSR_VEC: row blocks (row starts, nblocks + 1 entries) and split rows
+	spmvhip::DevBuf blocks;
+	long nblocks = 0;
+	int block_nnz = 512, split_nnz = 8192, lanes = 0;
+	spmvhip::DevBuf seg_desc;     // int32 triplets (row, j_begin, j_end) per segment
+	spmvhip::DevBuf seg_ptr;      // int32, nsplit + 1
+	spmvhip::DevBuf split_rows;   // int32, nsplit
+	spmvhip::DevBuf partials;     // double, nsegs
+	long nsplit = 0, nsegs = 0;
+
+	// SELL / SELL_C_S
+	spmvhip::DevBuf row_order;    // int32, m (SELL_C_S only)
+	spmvhip::DevBuf slice_w;      // int32, nslices
+	spmvhip::DevBuf slice_ptr;    // int64, nslices + 1
+	spmvhip::DevBuf s_cols;       // int32, padded
+	spmvhip::DevBuf s_vals;       // double, padded
+	long nslices = 0, padded = 0;
+
+	// DIV_RF
+	spmvhip::DevBuf div_stream;   // packet bytes
+	spmvhip::DevBuf div_pk_off;   // int64 per packet: byte offset
+	spmvhip::DevBuf div_groups;   // int32 per packet group: first packet
+	spmvhip::DevBuf div_group_rows; // int32 per packet group: [row_begin,row_end)
+	long div_npackets = 0, div_ngroups = 0, div_bytes = 0;
+	int div_lanes = 64;
+
+	// host-pointer contract
+	const double *bound_x = nullptr;
+	double *bound_y = nullptr;
+	spmvhip::DevBuf dx, dy;
+	bool y_on_device_only = false;
+
+	// statistics
+	double conv_ms = 0, mem_footprint = 0, csr_mem_footprint = 0;
+	double stat_ms_sum = 0;
+	long stat_calls = 0;
+};
+
+namespace spmvhip {
+
+// Per-kind conversion and SpMV entry points (csr.hip, sell.hip, div.hip).
+int csr_naive_convert(spmvhip_format *f);
+int csr_naive_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s);
+int csr_vec_convert(spmvhip_format *f);
+int csr_vec_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s);
+int sell_convert(spmvhip_format *f, bool sorted);
+int sell_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s);
+int div_convert(spmvhip_format *f, const int32_t *h_row_ptr, const int32_t *h_col,
+		const double *h_val);
+int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s);
+int div_export_decoded(const spmvhip_format *f, int which, void *buf, size_t *bytes);
+
+} // namespace spmvhip

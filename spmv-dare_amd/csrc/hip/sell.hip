@@ -1,0 +1,329 @@
+// SELL and SELL-C-sigma on gfx950, slice height C = 64 = one wavefront.
+//
+// Layout in HBM (column-major inside a slice, as the reference's
+// csr_to_sellcs writes it, src/spmv_kernels/sell-C-s/sellcs_format.c:173-186):
+//   entry k of slice row l lives at slice_ptr[s] + k*64 + l, so step k of a
+//   slice is one 512-byte coalesced load of values and a 256-byte load of
+//   column indices for the whole wave.
+//
+// csr_to_format runs on the device:
+//   SELL-C-sigma (sell_c_s.cpp:39-77 -> radix_sort.c:103-122, sellcs_format.c:137-200)
+//     1. row lengths;
+//     2. per sigma-window stable sort by descending length (ties keep the
+//        ascending row id) -- an LSD radix sort on 4-bit digits in LDS, one
+//        workgroup per window; the permutation equals the reference's;
+//     3. slice widths = longest row of the slice (the reference takes the
+//        first sorted row, which is the longest whenever sigma % 64 == 0);
+//     4. slice pointers = exclusive scan of width * 64;
+//     5. fill: one wave per slice, padding value 0.0 / column 0 (the
+//        reference memsets both arrays).
+//   SELL (sell.cpp:105-199, slice = 64 rows, no sort): width = longest row of
+//   the slice; padding value 0 with the column of the previous entry of the
+//   row, i.e. the row's last column, or for an empty row the last column of
+//   the nearest non-empty row above (0 if none) -- the reference with one
+//   OpenMP thread (its per-thread `col` restarts at 0 for each thread).
+//
+// spmv: one wave per slice, lane = row, one fused multiply-add per entry in
+// entry order (the reference's vfmacc, sellcs_mv_kernels_epi.c:292-297), then
+// y[row_order[row]] = acc for SELL-C-sigma and y[row] = acc for SELL. Only
+// rows < m are written.
+
+#include "format.hpp"
+#include "scan.hpp"
+
+namespace spmvhip {
+
+constexpr int kSliceC = 64;
+constexpr int kMaxSigma = 16384;
+constexpr int kSortThreads = 256;
+
+__global__ void row_lengths(const int32_t *__restrict__ rp, long m, int32_t *__restrict__ len)
+{
+	long r = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	if (r < m)
+		len[r] = rp[r + 1] - rp[r];
+}
+
+// Stable descending sort of one window of row lengths; writes the row ids.
+// LDS: two id arrays of `sigma` ints plus a 16 x 256 digit-count table.
+__global__ void __launch_bounds__(kSortThreads) sigma_window_sort(const int32_t *__restrict__ len, long m, int sigma,
+		int32_t *__restrict__ order)
+{
+	extern __shared__ int32_t lds[];
+	int32_t *ids = lds;
+	int32_t *tmp = lds + sigma;
+	int32_t *cnt = lds + 2 * sigma; // [16][kSortThreads]
+	__shared__ int32_t red[kSortThreads / kWave];
+	const long w0 = (long) blockIdx.x * sigma;
+	const int wl = (int) (m - w0 < sigma ? m - w0 : sigma);
+	const int t = threadIdx.x;
+	const int q = (wl + kSortThreads - 1) / kSortThreads;
+	const int i0 = t * q < wl ? t * q : wl;
+	const int i1 = (t + 1) * q < wl ? (t + 1) * q : wl;
+
+	int32_t mx = 0;
+	for (int i = t; i < wl; i += kSortThreads) {
+		ids[i] = (int32_t) (w0 + i);
+		int32_t k = len[w0 + i];
+		mx = k > mx ? k : mx;
+	}
+	for (int off = 32; off > 0; off >>= 1) {
+		int32_t o = __shfl_xor(mx, off, kWave);
+		mx = o > mx ? o : mx;
+	}
+	if (lane_id() == 0)
+		red[t / kWave] = mx;
+	__syncthreads();
+	mx = 0;
+	for (int wv = 0; wv < kSortThreads / kWave; wv++)
+		mx = red[wv] > mx ? red[wv] : mx;
+	int passes = 0;
+	for (uint32_t v = (uint32_t) mx; v; v >>= 4)
+		passes++;
+
+	for (int pass = 0, shift = 0; pass < passes; pass++, shift += 4) {
+		int c[16];
+#pragma unroll
+		for (int d = 0; d < 16; d++)
+			c[d] = 0;
+		for (int i = i0; i < i1; i++)
+			c[(len[ids[i]] >> shift) & 15]++;
+		// table in scan order: digit 15 first (descending), then thread id
+#pragma unroll
+		for (int d = 0; d < 16; d++)
+			cnt[(15 - d) * kSortThreads + t] = c[d];
+		__syncthreads();
+		// exclusive scan of the 16*256 table: thread t scans entries [16t, 16t+16)
+		int run = 0;
+		int loc[16];
+#pragma unroll
+		for (int k = 0; k < 16; k++) {
+			loc[k] = run;
+			run += cnt[16 * t + k];
+		}
+		int ex, tot;
+		block_scan<int, OpSum>(run, ex, tot);
+#pragma unroll
+		for (int k = 0; k < 16; k++)
+			cnt[16 * t + k] = ex + loc[k];
+		__syncthreads();
+#pragma unroll
+		for (int d = 0; d < 16; d++)
+			c[d] = cnt[(15 - d) * kSortThreads + t];
+		for (int i = i0; i < i1; i++) {
+			int32_t id = ids[i];
+			int d = (len[id] >> shift) & 15;
+			tmp[c[d]++] = id;
+		}
+		__syncthreads();
+		int32_t *sw = ids;
+		ids = tmp;
+		tmp = sw;
+	}
+	for (int i = t; i < wl; i += kSortThreads)
+		order[w0 + i] = ids[i];
+}
+
+// Slice widths (longest row) and slice sizes width * 64.
+__global__ void slice_widths(const int32_t *__restrict__ len, const int32_t *__restrict__ order, long m, long nslices,
+		int32_t *__restrict__ width, int64_t *__restrict__ size)
+{
+	long s = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= nslices)
+		return;
+	int w = 0;
+	for (int l = 0; l < kSliceC; l++) {
+		long r = s * kSliceC + l;
+		if (r >= m)
+			break;
+		int k = len[order ? order[r] : r];
+		w = k > w ? k : w;
+	}
+	width[s] = w;
+	size[s] = (int64_t) w * kSliceC;
+}
+
+// last_ne[r] = largest r' <= r with a non-empty row r' (or -1): input of the
+// SELL padding rule. Produced by an inclusive max-scan of these keys.
+__global__ void nonempty_keys(const int32_t *__restrict__ len, long m, int64_t *__restrict__ key)
+{
+	long r = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	if (r < m)
+		key[r] = len[r] > 0 ? r : -1;
+}
+
+__global__ void __launch_bounds__(kBlock) sell_fill(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+		const double *__restrict__ val, const int32_t *__restrict__ order, const int64_t *__restrict__ last_ne,
+		const int32_t *__restrict__ width, const int64_t *__restrict__ sptr, long m, long nslices,
+		int32_t *__restrict__ scol, double *__restrict__ sval)
+{
+	const long s = (long) blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+	if (s >= nslices)
+		return;
+	const int lane = lane_id();
+	const long r = s * kSliceC + lane;
+	const int w = width[s];
+	const int64_t base = sptr[s];
+	int j0 = 0, len = 0, pad_col = 0;
+	if (r < m) {
+		long src = order ? order[r] : r;
+		j0 = rp[src];
+		len = rp[src + 1] - j0;
+	}
+	if (!order) {
+		// SELL padding column: last column of this row or of the nearest non-empty row above
+		long rr = r < m ? r : m - 1;
+		int64_t ne = m > 0 ? last_ne[rr] : -1;
+		pad_col = ne >= 0 ? col[rp[ne + 1] - 1] : 0;
+	}
+	for (int k = 0; k < w; k++) {
+		int64_t idx = base + (int64_t) k * kSliceC + lane;
+		if (k < len) {
+			scol[idx] = col[j0 + k];
+			sval[idx] = val[j0 + k];
+		} else {
+			scol[idx] = pad_col;
+			sval[idx] = 0.0;
+		}
+	}
+}
+
+int sell_convert(spmvhip_format *f, bool sorted)
+{
+	hipStream_t s = f->stream;
+	const long m = f->m;
+	if (f->p.C != 0 && f->p.C != kSliceC)
+		return set_error(SPMVHIP_EINVAL, "sell: C = %d unsupported (slice height is one 64-lane wavefront)", f->p.C);
+	long sigma = f->p.sigma > 0 ? f->p.sigma : 4096;
+	if (sorted && sigma > kMaxSigma)
+		return set_error(SPMVHIP_EINVAL, "sell_c_s: sigma = %ld > %d (window must fit one workgroup's LDS)",
+				sigma, kMaxSigma);
+	f->p.C = kSliceC;
+	f->p.sigma = sorted ? sigma : 1;
+	f->nslices = (m + kSliceC - 1) / kSliceC;
+
+	DevBuf len, sizes;
+	SPMVHIP_TRY(len.alloc((m + 1) * sizeof(int32_t)));
+	SPMVHIP_TRY(sizes.alloc((f->nslices + 1) * sizeof(int64_t)));
+	row_lengths<<<grid_for(m, kBlock), kBlock, 0, s>>>(f->row_ptr.as<int32_t>(), m, len.as<int32_t>());
+	SPMVHIP_CHECK(hipGetLastError());
+
+	const int32_t *order = nullptr;
+	if (sorted) {
+		SPMVHIP_TRY(f->row_order.alloc(m * sizeof(int32_t)));
+		long nwin = (m + sigma - 1) / sigma;
+		size_t lds = (size_t) (2 * sigma + 16 * kSortThreads) * sizeof(int32_t);
+		SPMVHIP_CHECK(hipFuncSetAttribute((const void *) sigma_window_sort,
+				hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+		sigma_window_sort<<<(unsigned) nwin, kSortThreads, lds, s>>>(len.as<int32_t>(), m, (int) sigma,
+				f->row_order.as<int32_t>());
+		SPMVHIP_CHECK(hipGetLastError());
+		order = f->row_order.as<int32_t>();
+	}
+	SPMVHIP_TRY(f->slice_w.alloc(f->nslices * sizeof(int32_t)));
+	SPMVHIP_TRY(f->slice_ptr.alloc((f->nslices + 1) * sizeof(int64_t)));
+	slice_widths<<<grid_for(f->nslices, kBlock), kBlock, 0, s>>>(len.as<int32_t>(), order, m, f->nslices,
+			f->slice_w.as<int32_t>(), sizes.as<int64_t>());
+	SPMVHIP_CHECK(hipGetLastError());
+	SPMVHIP_TRY((device_scan<int64_t, OpSum>(sizes.as<int64_t>(), f->slice_ptr.as<int64_t>(), f->nslices, 0, 1, s)));
+	int64_t total = 0;
+	SPMVHIP_CHECK(hipMemcpyAsync(&total, f->slice_ptr.as<int64_t>() + f->nslices, sizeof(total),
+			hipMemcpyDeviceToHost, s));
+	SPMVHIP_CHECK(hipStreamSynchronize(s));
+	f->padded = total;
+
+	DevBuf last_ne;
+	if (!sorted) {
+		DevBuf keys;
+		SPMVHIP_TRY(keys.alloc(m * sizeof(int64_t)));
+		SPMVHIP_TRY(last_ne.alloc(m * sizeof(int64_t)));
+		nonempty_keys<<<grid_for(m, kBlock), kBlock, 0, s>>>(len.as<int32_t>(), m, keys.as<int64_t>());
+		SPMVHIP_CHECK(hipGetLastError());
+		SPMVHIP_TRY((device_scan<int64_t, OpMax>(keys.as<int64_t>(), last_ne.as<int64_t>(), m, 1, 0, s)));
+	}
+	SPMVHIP_TRY(f->s_cols.alloc(total * sizeof(int32_t)));
+	SPMVHIP_TRY(f->s_vals.alloc(total * sizeof(double)));
+	sell_fill<<<grid_for(f->nslices, kWavesPerBlock), kBlock, 0, s>>>(f->row_ptr.as<int32_t>(), f->col.as<int32_t>(),
+			f->val.as<double>(), order, sorted ? nullptr : last_ne.as<int64_t>(), f->slice_w.as<int32_t>(),
+			f->slice_ptr.as<int64_t>(), m, f->nslices, f->s_cols.as<int32_t>(), f->s_vals.as<double>());
+	SPMVHIP_CHECK(hipGetLastError());
+	SPMVHIP_CHECK(hipStreamSynchronize(s));
+	// the CSR copy is no longer needed once the slices exist
+	f->row_ptr.release();
+	f->col.release();
+	f->val.release();
+	f->mem_footprint = (double) total * 12 + (double) f->nslices * 12 + 8 + (sorted ? (double) m * 4 : 0);
+	return SPMVHIP_OK;
+}
+
+// One wave per slice. UNROLL entries are loaded before their FMAs so that
+// several loads are in flight per lane; the FMAs still run in entry order.
+template <int UNROLL>
+__global__ void __launch_bounds__(kBlock) sell_kernel(const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
+		const int32_t *__restrict__ scol, const double *__restrict__ sval, const int32_t *__restrict__ order,
+		const double *__restrict__ x, double *__restrict__ y, long m, long nslices, int remap)
+{
+	long blk = blockIdx.x;
+	if (remap) {
+		long per = gridDim.x / 8;
+		blk = (long) (blockIdx.x % 8) * per + blockIdx.x / 8;
+	}
+	const long s = blk * kWavesPerBlock + threadIdx.x / kWave;
+	if (s >= nslices)
+		return;
+	const int lane = lane_id();
+	const int w = width[s];
+	const double *v = sval + sptr[s] + lane;
+	const int32_t *c = scol + sptr[s] + lane;
+	double acc = 0;
+	int k = 0;
+	for (; k + UNROLL <= w; k += UNROLL) {
+		double vv[UNROLL];
+		int32_t cc[UNROLL];
+#pragma unroll
+		for (int u = 0; u < UNROLL; u++) {
+			vv[u] = __builtin_nontemporal_load(v + (long) (k + u) * kSliceC);
+			cc[u] = __builtin_nontemporal_load(c + (long) (k + u) * kSliceC);
+		}
+		double xx[UNROLL];
+#pragma unroll
+		for (int u = 0; u < UNROLL; u++)
+			xx[u] = x[cc[u]];
+#pragma unroll
+		for (int u = 0; u < UNROLL; u++)
+			acc = fma(vv[u], xx[u], acc);
+	}
+	for (; k < w; k++)
+		acc = fma(v[(long) k * kSliceC], x[c[(long) k * kSliceC]], acc);
+	const long r = s * kSliceC + lane;
+	if (r < m)
+		y[order ? order[r] : r] = acc;
+}
+
+int sell_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
+{
+	if (f->m == 0)
+		return SPMVHIP_OK;
+	unsigned grid = grid_for(f->nslices, kWavesPerBlock);
+	int remap = f->p.xcd_remap ? 1 : 0;
+	if (remap)
+		grid = (grid + 7) / 8 * 8;
+	const int32_t *order = f->kind == SPMVHIP_SELL_C_S ? f->row_order.as<int32_t>() : nullptr;
+	int u = f->p.unroll > 0 ? f->p.unroll : 4;
+#define LAUNCH_SELL(U)                                                                                       \
+	sell_kernel<U><<<grid, kBlock, 0, s>>>(f->slice_ptr.as<int64_t>(), f->slice_w.as<int32_t>(),            \
+			f->s_cols.as<int32_t>(), f->s_vals.as<double>(), order, x, y, f->m, f->nslices, remap)
+	if (u >= 8)
+		LAUNCH_SELL(8);
+	else if (u >= 4)
+		LAUNCH_SELL(4);
+	else if (u >= 2)
+		LAUNCH_SELL(2);
+	else
+		LAUNCH_SELL(1);
+#undef LAUNCH_SELL
+	SPMVHIP_CHECK(hipGetLastError());
+	return SPMVHIP_OK;
+}
+
+} // namespace spmvhip

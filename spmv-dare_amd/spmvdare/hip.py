@@ -1,0 +1,252 @@
+"""Python mirror of the reference's Matrix_Format plugin contract over the
+gfx950 C ABI (include/spmvhip.h).
+
+    fmt = csr_to_format(row_ptr, col_ind, values, m, n, nnz, symmetric=0,
+                        symmetry_expanded=0, kind="csr_vec")
+    fmt.spmv(x, y)                       # host arrays, y written in place
+    fmt.spmv_device(dx, dy)              # DeviceArray operands, asynchronous
+
+as src/spmv_kernels/spmv_kernel.h:8-29 declares them (csr_to_format,
+Matrix_Format::spmv / statistics_start / statistics_print_data,
+statistics_print_labels). Errors raise SpmvHipError carrying the C status,
+where the reference would exit().
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+_lib = _native.load("spmvhip")
+
+KINDS = {"csr_naive": 0, "csr_vec": 1, "sell": 2, "sell_c_s": 3, "div_rf": 4}
+KIND_NAMES = {v: k for k, v in KINDS.items()}
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("C", ctypes.c_int), ("sigma", ctypes.c_long),
+                ("lanes_per_row", ctypes.c_int), ("block_nnz", ctypes.c_int),
+                ("split_nnz", ctypes.c_int), ("xcd_remap", ctypes.c_int),
+                ("x_window", ctypes.c_int), ("unroll", ctypes.c_int),
+                ("div_threads", ctypes.c_int), ("div_lanes", ctypes.c_int),
+                ("div_packet_vals", ctypes.c_long), ("reserved", ctypes.c_int * 8)]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [("format_name", ctypes.c_char_p), ("m", ctypes.c_long), ("n", ctypes.c_long),
+                ("nnz", ctypes.c_long), ("mem_footprint", ctypes.c_double),
+                ("csr_mem_footprint", ctypes.c_double), ("conv_ms", ctypes.c_double),
+                ("padded_nnz", ctypes.c_long), ("num_work_items", ctypes.c_long),
+                ("num_split_rows", ctypes.c_long)]
+
+
+_vp = ctypes.c_void_p
+_sig = {
+    "spmvhip_init": ([ctypes.c_int], ctypes.c_int),
+    "spmvhip_device_name": ([ctypes.c_char_p, ctypes.c_long], ctypes.c_int),
+    "spmvhip_malloc": ([ctypes.POINTER(_vp), ctypes.c_size_t], ctypes.c_int),
+    "spmvhip_free": ([_vp], ctypes.c_int),
+    "spmvhip_memcpy_h2d": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
+    "spmvhip_memcpy_d2h": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
+    "spmvhip_memset": ([_vp, ctypes.c_int, ctypes.c_size_t], ctypes.c_int),
+    "spmvhip_synchronize": ([], ctypes.c_int),
+    "spmvhip_last_error": ([], ctypes.c_char_p),
+    "spmvhip_csr_to_format": ([_vp, _vp, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_long,
+                               ctypes.c_long, ctypes.c_long, ctypes.POINTER(Params),
+                               ctypes.POINTER(_vp)], ctypes.c_int),
+    "spmvhip_csr_to_format_device": ([_vp, _vp, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_long,
+                                      ctypes.POINTER(Params), ctypes.POINTER(_vp)], ctypes.c_int),
+    "spmvhip_spmv": ([_vp, _vp, _vp], ctypes.c_int),
+    "spmvhip_spmv_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "spmvhip_bind_host_xy": ([_vp, _vp, _vp], ctypes.c_int),
+    "spmvhip_sync_host_y": ([_vp], ctypes.c_int),
+    "spmvhip_statistics_start": ([_vp], ctypes.c_int),
+    "spmvhip_statistics_print_data": ([_vp, ctypes.c_char_p, ctypes.c_long], ctypes.c_int),
+    "spmvhip_statistics_print_labels": ([ctypes.c_char_p, ctypes.c_long], ctypes.c_int),
+    "spmvhip_get_info": ([_vp, ctypes.POINTER(Info)], ctypes.c_int),
+    "spmvhip_time_spmv": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp,
+                           ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "spmvhip_export": ([_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "spmvhip_destroy": ([_vp], None),
+}
+for _name, (_args, _res) in _sig.items():
+    _f = getattr(_lib, _name)
+    _f.argtypes = _args
+    _f.restype = _res
+
+EXPORTED = tuple(_sig)
+
+
+class SpmvHipError(RuntimeError):
+    def __init__(self, code, where):
+        msg = _lib.spmvhip_last_error()
+        super().__init__("%s failed (%d): %s" % (where, code, msg.decode() if msg else ""))
+        self.code = code
+
+
+def _check(rc, where):
+    if rc != 0:
+        raise SpmvHipError(rc, where)
+
+
+def init(device=0):
+    _check(_lib.spmvhip_init(device), "spmvhip_init")
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    _check(_lib.spmvhip_device_name(buf, 256), "spmvhip_device_name")
+    return buf.value.decode()
+
+
+def synchronize():
+    _check(_lib.spmvhip_synchronize(), "spmvhip_synchronize")
+
+
+class DeviceArray:
+    """A flat device allocation holding a numpy dtype (plumbing only)."""
+
+    def __init__(self, size, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.size = int(size)
+        self.nbytes = self.size * self.dtype.itemsize
+        p = _vp()
+        _check(_lib.spmvhip_malloc(ctypes.byref(p), max(self.nbytes, 8)), "spmvhip_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.size
+        _check(_lib.spmvhip_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes), "memcpy_h2d")
+
+    def download(self, out=None):
+        if out is None:
+            out = np.empty(self.size, dtype=self.dtype)
+        _check(_lib.spmvhip_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes), "memcpy_d2h")
+        return out
+
+    def fill_bytes(self, value):
+        _check(_lib.spmvhip_memset(self.ptr, value, self.nbytes), "memset")
+
+    def free(self):
+        if self.ptr:
+            _lib.spmvhip_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def make_params(kind, **kw):
+    p = Params()
+    p.kind = KINDS[kind] if isinstance(kind, str) else int(kind)
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError("unknown parameter %s" % k)
+        setattr(p, k, v)
+    return p
+
+
+class HipFormat:
+    """One converted matrix on the device (a Matrix_Format subclass instance)."""
+
+    def __init__(self, handle, params):
+        self._h = handle
+        self.params = params
+        info = self.info()
+        self.format_name = info["format_name"]
+        self.m, self.n, self.nnz = info["m"], info["n"], info["nnz"]
+        self.mem_footprint = info["mem_footprint"]
+        self.csr_mem_footprint = info["csr_mem_footprint"]
+
+    # -- Matrix_Format ---------------------------------------------------
+    def spmv(self, x, y):
+        """Host x (n) and y (m, written in place) as the reference driver passes them."""
+        assert x.dtype == np.float64 and y.dtype == np.float64
+        assert x.flags.c_contiguous and y.flags.c_contiguous
+        assert x.size >= self.n and y.size >= self.m
+        _check(_lib.spmvhip_spmv(self._h, x.ctypes.data, y.ctypes.data), "spmvhip_spmv")
+
+    def statistics_start(self):
+        _check(_lib.spmvhip_statistics_start(self._h), "statistics_start")
+
+    def statistics_print_data(self):
+        buf = ctypes.create_string_buffer(512)
+        _lib.spmvhip_statistics_print_data(self._h, buf, 512)
+        return buf.value.decode()
+
+    # -- device-resident path ---------------------------------------------
+    def spmv_device(self, dx, dy, stream=None):
+        _check(_lib.spmvhip_spmv_device(self._h, dx.ptr, dy.ptr, stream), "spmvhip_spmv_device")
+
+    def time(self, dx, dy, warmup, iters):
+        ms = (ctypes.c_float * max(iters, 1))()
+        tot = ctypes.c_float()
+        _check(_lib.spmvhip_time_spmv(self._h, dx.ptr, dy.ptr, warmup, iters, ms, ctypes.byref(tot)),
+               "spmvhip_time_spmv")
+        return np.array(ms[:iters], dtype=np.float64)
+
+    def info(self):
+        i = Info()
+        _check(_lib.spmvhip_get_info(self._h, ctypes.byref(i)), "get_info")
+        d = {f[0]: getattr(i, f[0]) for f in Info._fields_}
+        d["format_name"] = d["format_name"].decode()
+        return d
+
+    def export(self, which, dtype):
+        n = ctypes.c_size_t(0)
+        _check(_lib.spmvhip_export(self._h, which, None, ctypes.byref(n)), "export")
+        out = np.empty(n.value // np.dtype(dtype).itemsize, dtype=dtype)
+        n2 = ctypes.c_size_t(out.nbytes)
+        _check(_lib.spmvhip_export(self._h, which, out.ctypes.data, ctypes.byref(n2)), "export")
+        return out
+
+    def destroy(self):
+        if self._h:
+            _lib.spmvhip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def csr_to_format(row_ptr, col_ind, values, m, n, nnz, symmetric=0, symmetry_expanded=0,
+                  kind="csr_vec", **params):
+    """csr_to_format (src/spmv_kernels/spmv_kernel.h:28) for the chosen HIP format."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    ci = np.ascontiguousarray(col_ind, dtype=np.int32)
+    va = np.ascontiguousarray(values, dtype=np.float64)
+    p = make_params(kind, **params)
+    h = _vp()
+    rc = _lib.spmvhip_csr_to_format(rp.ctypes.data, ci.ctypes.data, va.ctypes.data, m, n, nnz,
+                                    symmetric, symmetry_expanded, ctypes.byref(p), ctypes.byref(h))
+    _check(rc, "csr_to_format(%s)" % kind)
+    return HipFormat(h.value, p)
+
+
+def csr_to_format_device(d_row_ptr, d_col_ind, d_values, m, n, nnz, kind="csr_vec", **params):
+    p = make_params(kind, **params)
+    h = _vp()
+    rc = _lib.spmvhip_csr_to_format_device(d_row_ptr.ptr, d_col_ind.ptr, d_values.ptr, m, n, nnz,
+                                           ctypes.byref(p), ctypes.byref(h))
+    _check(rc, "csr_to_format_device(%s)" % kind)
+    return HipFormat(h.value, p)
+
+
+def statistics_print_labels():
+    buf = ctypes.create_string_buffer(512)
+    _lib.spmvhip_statistics_print_labels(buf, 512)
+    return buf.value.decode()

@@ -1,0 +1,171 @@
+"""Write the golden fixtures from the reference's own code.
+
+Runs in the development container only (it needs the reference sources,
+compiled into oracle/_ref/ by `make -C oracle ref`). The outputs are data:
+hashes, descriptor statistics and a few small arrays produced by the
+reference's generator (artificial-matrix-generator/artificial_matrix_generation.c)
+and SELL-C-sigma conversion (src/spmv_kernels/sell-C-s/radix_sort.c,
+sellcs_format.c).
+
+    python tests/golden/make_fixtures.py
+
+Outputs (tests/golden/):
+    generator.json  per parameter line: m, n, nnz, FNV-1a 64 of row_ptr /
+                    col_ind / raw values, and the reference's descriptor stats
+    sellcs.json     per (matrix, C, sigma): FNV-1a 64 of each converted array
+    tiny_csr.npz, tiny_sellcs.npz   full arrays of the 1000 x 1000 test matrix
+"""
+import json
+import os
+import random
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+FIELDS = ["m", "n", "nnz", "h_row_ptr", "h_col_ind", "h_values", "avg_nnz_per_row",
+          "std_nnz_per_row", "min_nnz_per_row", "max_nnz_per_row", "skew", "avg_bw",
+          "avg_num_neighbours", "cross_row_similarity"]
+
+
+def fixed_lines():
+    return [
+        # matrix_generation_parameters/test.txt:1
+        ("test_txt", "1000 1000 5 1.6667 normal random 0.05 0 0.05 0.05 14"),
+        # artificial-matrix-generator/run.sh:12-16
+        ("run_sh_1", "65535 65535 5 1.6667 normal random 0.05 0 0.05 0.05 14"),
+        ("run_sh_2", "1082401 1082401 10 3.3333 normal random 0.3 10000 0.5 0.5 14"),
+        ("run_sh_3", "3020732 3020732 50 16.6667 normal random 0.3 100 1.9 0.05 14"),
+        # the benchmark's CPU-runnable and large single-matrix workloads
+        ("cfg1", "100000 100000 16 4 normal random 0.3 1.5 1 0.2 14"),
+        ("cfg2", "8388608 8388608 32 48 gamma diagonal 0.05 20 4 0.5 7"),
+        # edge cases of the generator's branches
+        ("simple_placement", "5000 3000 12 6 normal simple 0.2 2 1 0.3 5"),
+        ("bw_above_one", "4000 4000 20 5 gamma random 1.7 3 0.5 0.1 6"),
+        ("std_zero", "3000 3000 9 0 normal diagonal 0.1 0.5 0 0 7"),
+        ("skew_below_one", "3000 5000 9 3 gamma diagonal 0.02 0.5 1.5 0.9 8"),
+        ("degree_clamp", "2000 40 30 25 gamma random 1 5 2 0.5 9"),
+        ("min_rows", "100 100 4 2 normal random 0.5 1 1 0.5 10"),
+        ("dense_rows", "300 300 250 40 normal random 1 0.2 1 0.5 11"),
+        ("wide", "1000 200000 40 30 gamma diagonal 0.001 30 3 0.7 12"),
+        ("tall", "200000 1000 3 2 gamma random 0.5 10 0 0 13"),
+        ("neigh_two", "20000 20000 24 30 gamma diagonal 0.05 20 4 0.5 7"),
+    ]
+
+
+def sampled_lines(master_seed, count, rows_lo, rows_hi, avg_lo, avg_hi, skew_hi, tag):
+    """Parameter sets drawn from the batched workloads' ranges."""
+    rnd = random.Random(master_seed)
+    out = []
+    for k in range(count):
+        rows = int(round(10 ** rnd.uniform(np.log10(rows_lo), np.log10(rows_hi))))
+        avg = rnd.uniform(avg_lo, avg_hi)
+        std = avg * rnd.uniform(0.25, 2.0)
+        distr = rnd.choice(["normal", "gamma"])
+        place = rnd.choice(["random", "diagonal"])
+        bw = rnd.uniform(0.01, 1.0)
+        skew = rnd.uniform(1, skew_hi)
+        neigh = rnd.uniform(0, 8)
+        crs = rnd.uniform(0, 0.9)
+        seed = rnd.randrange(1, 2 ** 31)
+        line = "%d %d %.6g %.6g %s %s %.6g %.6g %.6g %.6g %d" % (
+            rows, rows, avg, std, distr, place, bw, skew, neigh, crs, seed)
+        out.append(("%s_%02d" % (tag, k), line))
+    return out
+
+
+def run_gen(line, dump=None):
+    args = [os.path.join(REF, "gen_dump")] + line.split() + ([dump] if dump else [])
+    out = subprocess.run(args, check=True, capture_output=True, text=True).stdout.split()
+    rec = {}
+    for k, v in zip(FIELDS, out):
+        rec[k] = int(v) if k in ("m", "n", "nnz") else (v if k.startswith("h_") else float(v))
+    return rec
+
+
+def read_sellcs(path, m):
+    with open(path, "rb") as f:
+        ns, total = struct.unpack("<qq", f.read(16))
+        order = np.frombuffer(f.read(8 * m), dtype=np.int64)
+        widths = np.frombuffer(f.read(8 * ns), dtype=np.int64)
+        sptr = np.frombuffer(f.read(8 * (ns + 1)), dtype=np.int64)
+        cols = np.frombuffer(f.read(8 * total), dtype=np.int64)
+        vals = np.frombuffer(f.read(8 * total), dtype=np.float64)
+    return order, widths, sptr, cols, vals
+
+
+def read_csr(path):
+    with open(path, "rb") as f:
+        m, n, nnz = struct.unpack("<qqq", f.read(24))
+        rp = np.frombuffer(f.read(4 * (m + 1)), dtype=np.int32)
+        ci = np.frombuffer(f.read(4 * nnz), dtype=np.int32)
+        va = np.frombuffer(f.read(8 * nnz), dtype=np.float64)
+    return rp, ci, va
+
+
+def digest(a):
+    """Short content digest used by the sellcs fixtures (sha256 of bytes)."""
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()[:32]
+
+
+def main():
+    if not os.path.exists(os.path.join(REF, "gen_dump")):
+        sys.exit("oracle/_ref/gen_dump missing: run `make -C oracle ref` first")
+    lines = fixed_lines()
+    lines += sampled_lines(21, 24, 1e3, 2e5, 4, 64, 100, "batch")
+    lines += sampled_lines(55, 8, 1e4, 4e5, 2, 128, 1000, "stream")
+    gen = {}
+    for name, line in lines:
+        rec = run_gen(line)
+        rec["line"] = line
+        gen[name] = rec
+        print(name, rec["nnz"], flush=True)
+    with open(os.path.join(HERE, "generator.json"), "w") as f:
+        json.dump(gen, f, indent=1, sort_keys=True)
+
+    sell = {}
+    with tempfile.TemporaryDirectory() as td:
+        # sigma = 1 (< C) is left out: the reference writes past its arrays there
+        # (sellcs_format.c:152 takes the first row of an unsorted slice as its
+        # width) and glibc aborts the run.
+        cases = [("test_txt", 64, 64), ("test_txt", 64, 256),
+                 ("test_txt", 64, 4096), ("test_txt", 256, 16384),
+                 ("cfg1", 64, 256), ("cfg1", 64, 4096), ("cfg1", 256, 16384),
+                 ("neigh_two", 64, 256), ("neigh_two", 64, 4096),
+                 ("batch_03", 64, 256), ("batch_07", 64, 4096), ("stream_01", 64, 4096),
+                 ("tall", 64, 128), ("dense_rows", 64, 64)]
+        dumped = {}
+        for name, C, sigma in cases:
+            line = gen[name]["line"]
+            if name not in dumped:
+                p = os.path.join(td, name + ".bin")
+                run_gen(line, p)
+                dumped[name] = p
+            out = os.path.join(td, "s.bin")
+            subprocess.run([os.path.join(REF, "sellcs_dump"), dumped[name], str(C), str(sigma), out],
+                           check=True, capture_output=True)
+            m = gen[name]["m"]
+            order, widths, sptr, cols, vals = read_sellcs(out, m)
+            key = "%s/C%d/s%d" % (name, C, sigma)
+            sell[key] = {"line": line, "C": C, "sigma": sigma, "total": int(sptr[-1]),
+                         "order": digest(order), "widths": digest(widths), "slice_ptr": digest(sptr),
+                         "cols": digest(cols), "vals": digest(vals)}
+            print(key, int(sptr[-1]), flush=True)
+            if name == "test_txt" and C == 64 and sigma == 256:
+                np.savez_compressed(os.path.join(HERE, "tiny_sellcs.npz"), order=order, widths=widths,
+                                    slice_ptr=sptr, cols=cols, vals=vals)
+        rp, ci, va = read_csr(dumped["test_txt"])
+        np.savez_compressed(os.path.join(HERE, "tiny_csr.npz"), row_ptr=rp, col_ind=ci, values=va)
+    with open(os.path.join(HERE, "sellcs.json"), "w") as f:
+        json.dump(sell, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,137 @@
+"""The native generator against the reference's own output (golden fixtures
+written by tests/golden/make_fixtures.py from the compiled reference
+generator), plus the benchmark input definitions (values, x)."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+from spmvdare import gen
+import pyoracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = json.load(open(os.path.join(HERE, "golden", "generator.json")))
+BIG = 40_000_000
+SMALL = sorted(n for n in FIX if FIX[n]["nnz"] < BIG)
+LARGE = sorted(n for n in FIX if FIX[n]["nnz"] >= BIG)
+
+
+def _check(name, stats_rtol=1e-9):
+    rec = FIX[name]
+    p = gen.GenParams.parse(rec["line"])
+    a = gen.generate(p, stats=True, map_values=False)
+    assert (a.m, a.n, a.nnz) == (rec["m"], rec["n"], rec["nnz"])
+    assert pyoracle.fnv1a(a.row_ptr) == rec["h_row_ptr"]
+    assert pyoracle.fnv1a(a.col_ind) == rec["h_col_ind"]
+    assert pyoracle.fnv1a(a.values) == rec["h_values"]
+    s = a.stats
+    for k in ("avg_nnz_per_row", "min_nnz_per_row", "max_nnz_per_row", "skew", "avg_bw"):
+        assert s[k] == pytest.approx(rec[k], rel=stats_rtol, abs=1e-12), k
+    assert s["std_nnz_per_row"] == pytest.approx(rec["std_nnz_per_row"], rel=1e-9)
+    assert s["avg_num_neighbours"] == pytest.approx(rec["avg_num_neighbours"], rel=1e-9, abs=1e-12)
+    if a.m >= 10000:
+        # below that the reference's cross-row statistic reads uninitialised
+        # heap memory (matrix_util.h:387-396 fills fewer slots than it averages)
+        assert s["cross_row_similarity"] == pytest.approx(rec["cross_row_similarity"], rel=1e-9, abs=1e-12)
+    return a
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_generator_matches_reference(name):
+    a = _check(name)
+    # structural invariants the reference guarantees
+    d = np.diff(a.row_ptr)
+    assert (d >= 0).all() and (d <= a.n).all()
+    if a.nnz:
+        assert a.col_ind.min() >= 0 and a.col_ind.max() < a.n
+        rows = np.repeat(np.arange(a.m), d)
+        inc = np.diff(a.col_ind)
+        same = rows[1:] == rows[:-1]
+        assert (inc[same] > 0).all(), "columns strictly increasing inside a row"
+        assert ((a.values >= 0) & (a.values < 1)).all()
+        k = a.values * 2.0 ** 31
+        assert (k == np.floor(k)).all(), "values are k / 2^31"
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("name", LARGE)
+def test_generator_matches_reference_large(name):
+    _check(name)
+
+
+def test_simple_placement_closed_form():
+    """placement 's' puts the nnz of every row at columns 0..d-1
+    (artificial_matrix_generation.c:387-395)."""
+    p = gen.GenParams.parse(FIX["simple_placement"]["line"])
+    a = gen.generate(p, stats=False, map_values=False)
+    for i in range(0, a.m, 97):
+        s, e = a.row_ptr[i], a.row_ptr[i + 1]
+        assert (a.col_ind[s:e] == np.arange(e - s)).all()
+
+
+def test_thread_count_independent():
+    import subprocess
+    import sys
+    line = FIX["batch_05"]["line"]
+    code = ("import sys; sys.path[:0]=%r; from spmvdare import gen; import pyoracle;"
+            "a=gen.generate(gen.GenParams.parse(%r),stats=False,map_values=False);"
+            "print(pyoracle.fnv1a(a.col_ind), pyoracle.fnv1a(a.values))") % (sys.path[:3], line)
+    outs = set()
+    for t in ("1", "3", "8"):
+        env = dict(os.environ, OMP_NUM_THREADS=t)
+        outs.add(subprocess.run([sys.executable, "-c", code], env=env, check=True,
+                                capture_output=True, text=True).stdout.strip())
+    assert len(outs) == 1
+
+
+def test_mapped_values_exact():
+    p = gen.GenParams.parse(FIX["test_txt"]["line"])
+    raw = gen.generate(p, stats=False, map_values=False).values
+    mapped = gen.generate(p, stats=False, map_values=True).values
+    assert (mapped == 2.0 * raw - 1.0).all()
+    assert mapped.min() >= -1.0 and mapped.max() < 1.0
+
+
+class _RandomData(ctypes.Structure):
+    _fields_ = [("fptr", ctypes.c_void_p), ("rptr", ctypes.c_void_p), ("state", ctypes.c_void_p),
+                ("rand_type", ctypes.c_int), ("rand_deg", ctypes.c_int), ("rand_sep", ctypes.c_int),
+                ("end_ptr", ctypes.c_void_p)]
+
+
+def test_x_definition_against_glibc():
+    """x_i = 2 * random_r()/2^31 - 1 from one 256-byte glibc stream, checked
+    against glibc called directly from Python."""
+    libc = ctypes.CDLL("libc.so.6")
+    seed = 15
+    state = ctypes.create_string_buffer(256)
+    rd = _RandomData()
+    assert libc.initstate_r(ctypes.c_uint(seed), state, ctypes.c_size_t(256), ctypes.byref(rd)) == 0
+    assert libc.srandom_r(ctypes.c_uint(seed), ctypes.byref(rd)) == 0
+    v = ctypes.c_int32()
+    expect = np.empty(5000)
+    for i in range(expect.size):
+        libc.random_r(ctypes.byref(rd), ctypes.byref(v))
+        expect[i] = 2.0 * (v.value / 2.0 ** 31) - 1.0
+    x = gen.make_x(expect.size, seed)
+    assert (x == expect).all()
+
+
+def test_bad_parameters_raise():
+    with pytest.raises(gen.GenError):
+        gen.generate(gen.GenParams(0, 10, 2, 1, "normal", "random", 0.5, 1, 0, 0, 1))
+
+
+def test_small_row_count_defined():
+    """nr_rows < 100 divides by zero in the reference (anchors_num = 0,
+    artificial_matrix_generation.c:263-268); here one anchor block is used."""
+    a = gen.generate(gen.GenParams(50, 50, 4, 1, "normal", "random", 0.5, 1, 1, 0.5, 3), stats=False)
+    assert a.m == 50 and a.nnz == a.row_ptr[-1]
+
+
+def test_degrees_match_generate():
+    p = gen.GenParams.parse(FIX["batch_02"]["line"])
+    d, nnz = gen.degrees(p)
+    a = gen.generate(p, stats=False)
+    assert nnz == a.nnz and (np.diff(a.row_ptr) == d).all()
