@@ -68,7 +68,7 @@ typedef struct spmvhip_params {
 	int C;                /* SELL slice height; must be 64 (one wavefront) -- default 64 */
 	long sigma;           /* SELL-C-sigma sort window, 1..16384 -- default 4096 */
 	int lanes_per_row;    /* CSR_VEC: 0 = auto; else lanes of the LDS reduction per row (1..64) */
-	int block_nnz;        /* CSR_VEC: nnz per row block staged in LDS (64..2048, default 512) */
+	int block_nnz;        /* CSR_VEC: nnz per row block staged in LDS (64..1024, default 512) */
 	int split_nnz;        /* CSR_VEC: rows longer than this are split across waves (default 8192) */
 	int xcd_remap;        /* 1 = give each XCD a contiguous range of work (default 0) */
 	int x_window;         /* SELL*: 1 = stage the slice's x window in LDS when it fits */
@@ -161,6 +161,19 @@ int spmvhip_time_spmv(spmvhip_format *f, const double *d_x, double *d_y,
 int spmvhip_export(const spmvhip_format *f, int which, void *buf, size_t *bytes_inout);
 
 void spmvhip_destroy(spmvhip_format *f);
+
+/* The host half of csr_to_format for DIV_RF, exposed so that the encoder can
+ * be checked without a GPU: encodes the CSR matrix into the reference's
+ * packet layout with T nnz-balanced partitions, L value lanes and packets
+ * of at most P values (src/spmv_kernels/DIV/div.cpp:236-513,
+ * div_kernels_rf.h:538-919). Two calls: with bytes == NULL only the sizes
+ * are returned in *nbytes, *npackets and *ngroups; then the caller passes
+ * buffers of those sizes: bytes[*nbytes], pk_off[*npackets + 1] (byte offset
+ * of each packet), grp_pk[*ngroups + 1] (first packet of each packet group).
+ * Needs no device. */
+int spmvhip_div_encode(const int32_t *row_ptr, const int32_t *col_ind, const double *values,
+		long m, long n, long nnz, int T, int L, long P, uint8_t *bytes, size_t *nbytes,
+		int64_t *pk_off, long *npackets, int32_t *grp_pk, long *ngroups);
 
 #ifdef __cplusplus
 }

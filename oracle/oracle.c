@@ -322,3 +322,77 @@ uint64_t oracle_fnv1a(const void *p, long nbytes)
 	}
 	return h;
 }
+
+/* ------------------------------------------------------------ DIV-RF decode */
+
+static uint64_t le_bytes(const unsigned char *p, long n)
+{
+	uint64_t v = 0;
+	for (long b = 0; b < n; b++)
+		v |= (uint64_t) p[b] << (8 * b);
+	return v;
+}
+
+static uint32_t le32(const unsigned char *p)
+{
+	return (uint32_t) le_bytes(p, 4);
+}
+
+long oracle_div_decode(const unsigned char *bytes, long nbytes, long L, int32_t *rows, int32_t *cols,
+		double *vals, long cap)
+{
+	long at = 0, out = 0;
+	while (at < nbytes) {
+		const unsigned char *pk = bytes + at;
+		long nv = le32(pk), nu = le32(pk + 4);
+		long row_min = le32(pk + 12), col_min = le32(pk + 16);
+		long row_bits = pk[20], col_bits = pk[21], nrfs = le32(pk + 24);
+		long cb = (row_bits + col_bits + 7) >> 3;
+		const unsigned char *rfs = pk + 28 + 4 * L;
+		const unsigned char *coords = rfs + 8 * nrfs;
+		const unsigned char *lens = coords + nv * cb;
+		const unsigned char *streams = lens + nu;
+		long lane_start[4096], cursor[4096], total = 0;
+		if (L > 4096 || out + nv > cap)
+			return -1;
+		for (long l = 0; l < L; l++) {
+			lane_start[l] = total;
+			cursor[l] = 0;
+			total += le32(pk + 28 + 4 * l);
+		}
+		long ks = 0, cs = 0;
+		for (long g = 0; g < nrfs; g++) {
+			long rf = le32(rfs + 4 * g), n = le32(rfs + 4 * (nrfs + g));
+			uint64_t prev[4096];
+			for (long l = 0; l < L; l++)
+				prev[l] = 0;
+			/* position q of the group: lane q % L; its coordinates follow the
+			 * element-wise interleave of lib/macros/permutation.h:47-111 */
+			long dv = n / L, mul_c = dv * rf * L;
+			for (long q = 0; q < n; q++) {
+				long lane = q % L, pp = q / L;
+				unsigned lb = lens[ks + q];
+				long len = lb & 15, tz = (lb >> 4) << 2;
+				if (len) {
+					prev[lane] += le_bytes(streams + lane_start[lane] + cursor[lane], len) << tz;
+					cursor[lane] += len;
+				}
+				double d;
+				memcpy(&d, &prev[lane], 8);
+				for (long t = 0; t < rf; t++) {
+					long cpos = pp < dv ? (pp * rf + t) * L + lane : mul_c + lane * rf + t;
+					uint64_t c = le_bytes(coords + (cs + cpos) * cb, cb);
+					uint64_t mask = row_bits ? ((1ull << row_bits) - 1) : 0;
+					rows[out + cs + cpos] = (int32_t) (row_min + (long) (c & mask));
+					cols[out + cs + cpos] = (int32_t) (col_min + (long) (c >> row_bits));
+					vals[out + cs + cpos] = d;
+				}
+			}
+			ks += n;
+			cs += n * rf;
+		}
+		out += nv;
+		at += 28 + 4 * L + 8 * nrfs + nv * cb + nu + total;
+	}
+	return out;
+}

@@ -89,6 +89,14 @@ void oracle_sell_fill(const int32_t *rp, const int32_t *col, const double *val, 
 void oracle_sell_spmv(long m, long VL, const int64_t *cluster_ptr, const int64_t *ja,
 		const double *a, const double *x, double *y);
 
+/* Decode a DIV-RF packet stream with L value lanes (the inverse of
+ * compress_kernel_div, src/spmv_kernels/DIV/div_kernels_rf.h:538-919, as the
+ * reference's decompress path reads it, :948-1162 and :1247-1267): writes one
+ * (row, col, value) per stored coordinate, packet after packet, in the
+ * packet's coordinate order. Returns the number written, -1 on overflow. */
+long oracle_div_decode(const unsigned char *bytes, long nbytes, long L, int32_t *rows, int32_t *cols,
+		double *vals, long cap);
+
 /* FNV-1a 64 of a byte range: the hash the generator fixtures record. */
 uint64_t oracle_fnv1a(const void *p, long nbytes);
 

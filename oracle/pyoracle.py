@@ -159,3 +159,48 @@ def dd_rel_err(y, hi, lo):
     d = (np.asarray(y) - hi) - lo
     r = np.linalg.norm(hi)
     return float(np.linalg.norm(d) / r) if r > 0 else float(np.linalg.norm(d))
+
+
+_lib.oracle_div_decode.argtypes = [_vp, _l, _l, _vp, _vp, _vp, _l]
+_lib.oracle_div_decode.restype = _l
+
+
+def div_decode(stream, L, nnz):
+    """(rows, cols, vals) of a DIV-RF packet stream, packet by packet."""
+    b = np.ascontiguousarray(stream, dtype=np.uint8)
+    rows = np.empty(max(nnz, 1), dtype=np.int32)
+    cols = np.empty(max(nnz, 1), dtype=np.int32)
+    vals = np.empty(max(nnz, 1), dtype=np.float64)
+    k = _lib.oracle_div_decode(b.ctypes.data, b.size, L, rows.ctypes.data, cols.ctypes.data,
+                               vals.ctypes.data, nnz)
+    if k < 0:
+        raise ValueError("malformed DIV stream")
+    return rows[:k], cols[:k], vals[:k]
+
+
+def div_packet_sizes(stream, L):
+    """Byte length of each packet, read from the headers."""
+    b = np.ascontiguousarray(stream, dtype=np.uint8)
+    out = []
+    at = 0
+    while at < b.size:
+        h = b[at:at + 28 + 4 * L].view(np.uint32) if (28 + 4 * L) % 4 == 0 else None
+        nv, nu, nrfs = int(h[0]), int(h[1]), int(h[6])
+        rb, cbits = int(b[at + 20]), int(b[at + 21])
+        cb = (rb + cbits + 7) >> 3
+        lanes = int(h[7:7 + L].astype(np.int64).sum())
+        size = 28 + 4 * L + 8 * nrfs + nv * cb + nu + lanes
+        out.append(size)
+        at += size
+    return out
+
+
+def div_mask_padding(stream, L):
+    """Copy of a stream with each header's two padding bytes (offsets 22-23)
+    zeroed: the reference leaves them uninitialised."""
+    b = np.array(stream, dtype=np.uint8, copy=True)
+    at = 0
+    for s in div_packet_sizes(b, L):
+        b[at + 22:at + 24] = 0
+        at += s
+    return b

@@ -1,0 +1,362 @@
+// DIV-RF packet encoder (layout and reference citations: div_codec.hpp).
+// Packet boundaries are found sequentially per partition (they chain), then
+// the packets are encoded in parallel and concatenated in partition order.
+
+#include "div_codec.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <omp.h>
+
+namespace spmvdiv {
+namespace {
+
+// Number of bits of the range [0, v] (0 for v == 0), lib/bit_ops.h:225-243.
+inline uint64_t req_bits(uint64_t v)
+{
+	return v == 0 ? 0 : 64 - __builtin_clzll(v);
+}
+
+// Closest prefix-sum index, ties to the later index (lib/macros/macrolib.h:537-575).
+long closest(const int32_t *A, long lo, long hi, int64_t target)
+{
+	long s = lo, e = hi;
+	if (target < A[s])
+		return s;
+	if (target > A[e])
+		return e;
+	for (;;) {
+		long mid = (s + e) / 2;
+		if (mid == s || mid == e)
+			break;
+		if (target > A[mid])
+			s = mid;
+		else
+			e = mid;
+	}
+	if (A[s] == target)
+		return s;
+	if (A[e] == target)
+		return e;
+	return (target - A[s]) < (A[e] - target) ? s : e;
+}
+
+// nnz-balanced row range of partition t of T (lib/parallel_util.h:156-184 on row_ptr).
+void partition(long T, long t, const int32_t *rp, long m, long nnz, long *s, long *e)
+{
+	int64_t target = rp[0] + (nnz * t) / T;
+	int64_t target_next = rp[0] + (nnz * (t + 1)) / T;
+	*s = (t == 0) ? 0 : closest(rp, 0, m - 1, target);
+	*e = (t == T - 1) ? m : closest(rp, 0, m - 1, target_next);
+}
+
+// Packet size with a complete last row where possible, at most 255 rows
+// (div_kernels_rf.h:437-464 with force_row_index_lte_1_byte).
+long select_num_vals(const int32_t *rp, long i_s, long j_s, long num_vals)
+{
+	long j_e = j_s + num_vals, i_e = i_s;
+	while (rp[i_e] < j_e)
+		i_e++;
+	if (j_e != rp[i_e] && i_e - 1 > i_s) {
+		i_e--;
+		j_e = rp[i_e];
+	}
+	if (i_e - i_s > 255)
+		i_e = i_s + 255;
+	long r = rp[i_e] - j_s;
+	return r > num_vals ? num_vals : r;
+}
+
+// Interleave N elements (units of U) into S segments: element-wise for the
+// full rounds, unit-wise for the remainder (lib/macros/permutation.h:47-111
+// with solid_units_main = 0, solid_units_rem = 1).
+template <class T>
+void interleave(const T *in, T *out, long N, long S, long U)
+{
+	long units = N / U, rem = N - units * U;
+	long div = units / S, mod = units % S;
+	long mul = div * U * S;
+	std::vector<long> seg(S);
+	for (long s = 0; s < S; s++) {
+		seg[s] = U * (s * div + (s < mod ? s : mod));
+		if (s > mod)
+			seg[s] += rem;
+	}
+	for (long j = 0; j < mul; j++)
+		out[j] = in[seg[j % S] + j / S];
+	for (long j = mul; j < N; j++)
+		out[j] = in[seg[(j / U) % S] + U * (j / (U * S)) + j % U];
+}
+
+struct Key {
+	uint64_t v;
+	uint32_t row, col;
+	bool operator<(const Key &o) const
+	{
+		if (v != o.v)
+			return v < o.v;
+		if (row != o.row)
+			return row < o.row;
+		return col < o.col;
+	}
+};
+
+void put32(std::vector<uint8_t> &b, size_t at, uint32_t v)
+{
+	memcpy(&b[at], &v, 4);
+}
+
+// One packet (div_kernels_rf.h:538-919, lossless: DIV_VC_TOLERANCE = 0).
+int compress(const int32_t *rp, const int32_t *col, const double *val, long i0, long j0, long nv, int L,
+		std::vector<uint8_t> &out, const char **err)
+{
+	std::vector<Key> key(nv);
+	long i = i0;
+	uint32_t cmin = (uint32_t) col[j0], cmax = cmin;
+	for (long k = 0; k < nv; k++) {
+		long j = j0 + k;
+		while (j >= rp[i + 1])
+			i++;
+		uint64_t u;
+		memcpy(&u, &val[j], 8);
+		uint32_t c = (uint32_t) col[j];
+		cmin = c < cmin ? c : cmin;
+		cmax = c > cmax ? c : cmax;
+		key[k] = {u, (uint32_t) i, c};
+	}
+	uint64_t row_bits = req_bits((uint64_t) (i - i0));
+	if (row_bits > 8) {
+		*err = "row_bits > 8";
+		return -1;
+	}
+	uint64_t col_bits = req_bits(cmax - cmin);
+	uint64_t cb = (row_bits + col_bits + 7) >> 3;
+	if (row_bits % 8) {
+		uint64_t cmpl = 8 - (row_bits % 8);
+		if ((((row_bits + cmpl) + col_bits + 7) >> 3) == cb)
+			row_bits += cmpl;
+	}
+	col_bits = (cb << 3) - row_bits;
+	if (col_bits > 32) {
+		*err = "col_bits > 32";
+		return -1;
+	}
+	const uint64_t num_rows = (uint64_t) (i - i0) + 1;
+
+	std::sort(key.begin(), key.end());
+
+	// replication factor of every element: length of its run of equal value bits
+	std::vector<int32_t> rf(nv);
+	long max_rf = 1, nuniq = 0;
+	for (long a = 0; a < nv;) {
+		long b = a + 1;
+		while (b < nv && key[b].v == key[a].v)
+			b++;
+		for (long t = a; t < b; t++)
+			rf[t] = (int32_t) (b - a);
+		max_rf = std::max(max_rf, b - a);
+		nuniq++;
+		a = b;
+	}
+	// stable counting sort by replication factor (bucketsort_gen.c:164-193)
+	std::vector<long> start(max_rf + 2, 0);
+	for (long k = 0; k < nv; k++)
+		start[rf[k] + 1]++;
+	for (long b = 1; b < max_rf + 2; b++)
+		start[b] += start[b - 1];
+	std::vector<long> fill(start.begin(), start.end() - 1);
+	std::vector<Key> sk(nv);
+	for (long k = 0; k < nv; k++)
+		sk[fill[rf[k]]++] = key[k];
+
+	// unique values in that order, then per replication factor: interleave
+	// the values and (in units of rf) their coordinates over the L lanes
+	std::vector<uint64_t> uniq(nuniq), uniq_il(nuniq);
+	std::vector<Key> sk_il(nv);
+	std::vector<uint32_t> rfs, per_rf;
+	long u0 = 0;
+	for (long r = 1; r <= max_rf; r++) {
+		long ls = start[r], le = start[r + 1];
+		if (le == ls)
+			continue;
+		long n_unique = (le - ls) / r;
+		for (long t = 0; t < n_unique; t++) // bucket r holds runs of r equal values
+			uniq[u0 + t] = sk[ls + t * r].v;
+		rfs.push_back((uint32_t) r);
+		per_rf.push_back((uint32_t) n_unique);
+		interleave(&uniq[u0], &uniq_il[u0], n_unique, L, 1);
+		interleave(&sk[ls], &sk_il[ls], le - ls, L, r);
+		u0 += n_unique;
+	}
+	const long num_rfs = (long) rfs.size();
+
+	// lane streams: per rf group the lane's previous value restarts at 0
+	std::vector<std::vector<uint8_t>> lanes(L);
+	std::vector<uint8_t> lens(nuniq);
+	u0 = 0;
+	for (long g = 0; g < num_rfs; g++) {
+		std::vector<uint64_t> prev(L, 0);
+		for (long q = 0; q < (long) per_rf[g]; q++) {
+			long k = u0 + q;
+			int lane = (int) (q % L);
+			uint64_t diff = uniq_il[k] - prev[lane];
+			prev[lane] += diff;
+			if (diff == 0) {
+				lens[k] = 0;
+				continue;
+			}
+			uint64_t lz = __builtin_clzll(diff), tz = __builtin_ctzll(diff);
+			uint64_t rem_tz = tz & 3;
+			uint64_t len_bits = 64 - lz - tz;
+			uint64_t len = (len_bits + 7) >> 3;
+			uint64_t rem = (len << 3) - len_bits;
+			if (rem >= rem_tz) {
+				rem -= rem_tz;
+				tz -= rem_tz;
+			}
+			if (rem <= lz) {
+				lz -= rem;
+				rem = 0;
+			} else {
+				rem -= lz;
+				lz = 0;
+			}
+			tz -= rem;
+			uint64_t tz4 = tz >> 2;
+			tz = tz4 << 2;
+			len_bits = 64 - lz - tz;
+			len = (len_bits + 7) >> 3;
+			uint64_t payload = diff >> tz;
+			for (uint64_t b = 0; b < len; b++)
+				lanes[lane].push_back((uint8_t) (payload >> (8 * b)));
+			lens[k] = (uint8_t) (len | (tz4 << 4));
+		}
+		u0 += per_rf[g];
+	}
+
+	// assemble
+	size_t hb = (size_t) header_bytes(L);
+	size_t total = hb + 8 * (size_t) num_rfs + (size_t) nv * cb + (size_t) nuniq;
+	for (int l = 0; l < L; l++)
+		total += lanes[l].size();
+	out.assign(total, 0);
+	put32(out, 0, (uint32_t) nv);
+	put32(out, 4, (uint32_t) nuniq);
+	put32(out, 8, (uint32_t) num_rows);
+	put32(out, 12, (uint32_t) i0);
+	put32(out, 16, cmin);
+	out[20] = (uint8_t) row_bits;
+	out[21] = (uint8_t) col_bits;
+	put32(out, 24, (uint32_t) num_rfs);
+	for (int l = 0; l < L; l++)
+		put32(out, kHeaderFixed + 4 * l, (uint32_t) lanes[l].size());
+	size_t at = hb;
+	for (long g = 0; g < num_rfs; g++)
+		put32(out, at + 4 * g, rfs[g]);
+	at += 4 * num_rfs;
+	for (long g = 0; g < num_rfs; g++)
+		put32(out, at + 4 * g, per_rf[g]);
+	at += 4 * num_rfs;
+	for (long k = 0; k < nv; k++) {
+		uint64_t c = (uint64_t) (sk_il[k].row - (uint32_t) i0) | ((uint64_t) (sk_il[k].col - cmin) << row_bits);
+		for (uint64_t b = 0; b < cb; b++)
+			out[at + k * cb + b] = (uint8_t) (c >> (8 * b));
+	}
+	at += nv * cb;
+	memcpy(&out[at], lens.data(), nuniq);
+	at += nuniq;
+	for (int l = 0; l < L; l++) {
+		if (!lanes[l].empty())
+			memcpy(&out[at], lanes[l].data(), lanes[l].size());
+		at += lanes[l].size();
+	}
+	return 0;
+}
+
+} // namespace
+
+int encode(const int32_t *rp, const int32_t *col, const double *val, long m, long n, long nnz, int T, int L,
+		long P, Stream *out, const char **err)
+{
+	(void) n;
+	*err = nullptr;
+	out->T = T;
+	out->L = L;
+	out->P = P;
+	out->bytes.clear();
+	out->pk_off.assign(1, 0);
+	out->pk_nnz.clear();
+	out->grp_pk.clear();
+	out->grp_row.clear();
+
+	struct Plan {
+		long i, j, nv;
+		bool group_start;
+	};
+	std::vector<Plan> plan;
+	std::vector<long> grp_first_row;
+	for (long t = 0; t < T && m > 0; t++) {
+		long i_s, i_e;
+		partition(T, t, rp, m, nnz, &i_s, &i_e);
+		long j = rp[i_s], j_e = rp[i_e];
+		long i = i_s;
+		bool first = true;
+		while (j < j_e) {
+			while (rp[i] < j)
+				i++;
+			if (rp[i] != j)
+				i--;
+			while (rp[i] == rp[i + 1]) // empty rows are skipped
+				i++;
+			long nmax = (j + P <= j_e) ? P : j_e - j;
+			long nv = select_num_vals(rp, i, j, nmax);
+			bool gs = (j == rp[i]);
+			plan.push_back({i, j, nv, gs});
+			if (gs)
+				grp_first_row.push_back(first ? i_s : i);
+			first = false;
+			j += nv;
+		}
+	}
+	const long np = (long) plan.size();
+	std::vector<std::vector<uint8_t>> pk(np);
+	int fail = 0;
+	const char *ferr = nullptr;
+#pragma omp parallel for schedule(dynamic, 8)
+	for (long p = 0; p < np; p++) {
+		const char *e = nullptr;
+		if (compress(rp, col, val, plan[p].i, plan[p].j, plan[p].nv, L, pk[p], &e)) {
+#pragma omp critical
+			{
+				fail = 1;
+				ferr = e;
+			}
+		}
+	}
+	if (fail) {
+		*err = ferr;
+		return -1;
+	}
+	size_t total = 0;
+	for (long p = 0; p < np; p++)
+		total += pk[p].size();
+	out->bytes.resize(total);
+	size_t at = 0;
+	for (long p = 0; p < np; p++) {
+		if (!pk[p].empty())
+			memcpy(&out->bytes[at], pk[p].data(), pk[p].size());
+		at += pk[p].size();
+		out->pk_off.push_back((int64_t) at);
+		out->pk_nnz.push_back(plan[p].j);
+		if (plan[p].group_start)
+			out->grp_pk.push_back((int32_t) p);
+	}
+	out->pk_nnz.push_back(nnz);
+	out->grp_pk.push_back((int32_t) np);
+	for (size_t g = 0; g < grp_first_row.size(); g++)
+		out->grp_row.push_back(g == 0 ? 0 : (int32_t) grp_first_row[g]);
+	out->grp_row.push_back((int32_t) m);
+	return 0;
+}
+
+} // namespace spmvdiv

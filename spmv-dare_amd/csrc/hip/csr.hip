@@ -134,8 +134,8 @@ int csr_vec_convert(spmvhip_format *f)
 	const long m = f->m;
 	f->block_nnz = f->p.block_nnz > 0 ? f->p.block_nnz : 512;
 	f->split_nnz = f->p.split_nnz > 0 ? f->p.split_nnz : 8192;
-	if (f->block_nnz < 64 || f->block_nnz > 2048)
-		return set_error(SPMVHIP_EINVAL, "csr_vec: block_nnz %d outside [64, 2048]", f->block_nnz);
+	if (f->block_nnz < 64 || f->block_nnz > 1024)
+		return set_error(SPMVHIP_EINVAL, "csr_vec: block_nnz %d outside [64, 1024]", f->block_nnz);
 	if (f->split_nnz < f->block_nnz)
 		return set_error(SPMVHIP_EINVAL, "csr_vec: split_nnz %d < block_nnz %d", f->split_nnz, f->block_nnz);
 	int L = f->p.lanes_per_row;
@@ -185,6 +185,8 @@ int csr_vec_convert(spmvhip_format *f)
 		csr_split_desc<<<grid_for(ns, kBlock), kBlock, 0, s>>>(f->row_ptr.as<int32_t>(), f->split_rows.as<int32_t>(),
 				ns, f->split_nnz, segpos.as<int64_t>(), f->seg_ptr.as<int32_t>(), f->seg_desc.as<int32_t>());
 		SPMVHIP_CHECK(hipGetLastError());
+		// cnt / segpos are released at the end of this block
+		SPMVHIP_CHECK(hipStreamSynchronize(s));
 	}
 	SPMVHIP_CHECK(hipStreamSynchronize(s));
 	f->mem_footprint = f->csr_mem_footprint + (double) (nb + 1) * 4 + (double) f->nsegs * 20;
@@ -317,10 +319,8 @@ int csr_vec_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 		LAUNCH_BINNED(256);
 	else if (f->block_nnz <= 512)
 		LAUNCH_BINNED(512);
-	else if (f->block_nnz <= 1024)
-		LAUNCH_BINNED(1024);
 	else
-		LAUNCH_BINNED(2048);
+		LAUNCH_BINNED(1024);
 #undef LAUNCH_BINNED
 	SPMVHIP_CHECK(hipGetLastError());
 	if (f->nsegs > 0) {

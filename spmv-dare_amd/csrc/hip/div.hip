@@ -1,16 +1,274 @@
-// DIV-RF on gfx950 (placeholder until the codec lands).
+// DIV-RF on gfx950: host encoding (csrc/div/div_codec.cpp, the reference's
+// packet layout) and a device decoder that computes y = A x straight from the
+// packet stream (the CPU kernel it replaces: decompress_and_compute_kernel_div,
+// src/spmv_kernels/DIV/div_kernels_rf.h:948-1162, driven by compute_div,
+// src/spmv_kernels/DIV/div.cpp:1201-1316).
+//
+// Device layout: the packets of the reference stream, each starting at a
+// 16-byte aligned offset (the bytes of a packet are unchanged), plus
+//   pk_off[p]   byte offset of packet p       pk_nnz[p]  its first nnz index
+//   grp_pk[g]   first packet of packet group g (a group starts at a row start)
+//   grp_row[g]  first row owned by group g: group g writes every y row in
+//               [grp_row[g], grp_row[g+1]), empty rows included, so no
+//               separate zeroing pass is needed.
+//
+// Decoder: one wavefront per packet group. The L <= 64 value streams of a
+// packet map to lanes 0..L-1; a wave-level exclusive scan of the header's
+// lane sizes gives each lane the start of its stream. Per replication-factor
+// group a lane walks its values in order (the reference's interleave puts
+// value p of lane s at position p*L + s), adds each decoded delta
+// (bits << trailing_zeros) to its running value, and for each of the rf
+// coordinates of that value (element-interleaved, so the lanes read them
+// coalesced) accumulates value * x[col] into an LDS copy of the packet's
+// <= 255 rows. At the end of the packet the LDS rows are added to y.
+
 #include "format.hpp"
+#include "../div/div_codec.hpp"
+
+#include <cstring>
+#include <vector>
+
 namespace spmvhip {
-int div_convert(spmvhip_format *, const int32_t *, const int32_t *, const double *)
+
+constexpr int kDivWavesPerBlock = kWavesPerBlock;
+
+__device__ __forceinline__ uint64_t load_le(const uint8_t *p, int nbytes)
 {
-	return set_error(SPMVHIP_EINVAL, "div_rf: not available in this build");
+	uint64_t v = 0;
+	for (int b = 0; b < nbytes; b++)
+		v |= (uint64_t) p[b] << (8 * b);
+	return v;
 }
-int div_spmv(spmvhip_format *, const double *, double *, hipStream_t)
+
+__device__ __forceinline__ void wave_fence()
 {
-	return set_error(SPMVHIP_EINVAL, "div_rf: not available in this build");
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+	__builtin_amdgcn_wave_barrier();
 }
-int div_export_decoded(const spmvhip_format *, int, void *, size_t *)
+
+// MODE 0: y += decoded products (spmv). MODE 1: write the decoded
+// coordinates and values in packet order (rows/cols/vals at pk_nnz + index).
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) div_kernel(const uint8_t *__restrict__ bytes, const int64_t *__restrict__ pk_off,
+		const int64_t *__restrict__ pk_nnz, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
+		long ngroups, int L, const double *__restrict__ x, double *__restrict__ y, int32_t *__restrict__ rows_out,
+		int32_t *__restrict__ cols_out, double *__restrict__ vals_out)
 {
-	return set_error(SPMVHIP_EINVAL, "div_rf: not available in this build");
+	__shared__ double yl[kDivWavesPerBlock][256];
+	const int lane = lane_id(), wv = threadIdx.x / kWave;
+	const long g = (long) blockIdx.x * kDivWavesPerBlock + wv;
+	if (g >= ngroups)
+		return; // waves do not synchronise with each other
+	if (MODE == 0) {
+		for (long r = grp_row[g] + lane; r < grp_row[g + 1]; r += kWave)
+			y[r] = 0.0;
+		wave_fence();
+	}
+	for (int p = grp_pk[g]; p < grp_pk[g + 1]; p++) {
+		const uint8_t *pk = bytes + pk_off[p];
+		const uint32_t *h = (const uint32_t *) pk;
+		const uint32_t nv = h[0], nu = h[1], nrows = h[2], row_min = h[3], col_min = h[4];
+		const int row_bits = pk[20], col_bits = pk[21];
+		const uint32_t nrfs = h[6];
+		const int cb = (row_bits + col_bits + 7) >> 3;
+		const uint64_t row_mask = row_bits ? ((1ull << row_bits) - 1) : 0;
+		// start of this lane's value stream: wave exclusive scan of the lane sizes
+		long my_size = lane < L ? (long) h[7 + lane] : 0;
+		long incl = my_size;
+		for (int off = 1; off < kWave; off <<= 1) {
+			long o = __shfl_up(incl, off, kWave);
+			if (lane >= off)
+				incl += o;
+		}
+		const long lane_base = incl - my_size;
+		const uint32_t *rfs = h + 7 + L;
+		const uint32_t *per_rf = rfs + nrfs;
+		const uint8_t *coords = pk + 28 + 4 * L + 8 * (long) nrfs;
+		const uint8_t *lens = coords + (long) nv * cb;
+		const uint8_t *stream = lens + nu + lane_base;
+		const long nnz0 = MODE == 1 ? pk_nnz[p] : 0;
+
+		if (MODE == 0) {
+			for (uint32_t r = lane; r < nrows; r += kWave)
+				yl[wv][r] = 0.0;
+			wave_fence();
+		}
+		long cursor = 0, ks = 0, cs = 0;
+		for (uint32_t gi = 0; gi < nrfs; gi++) {
+			const long rf = rfs[gi], n = per_rf[gi];
+			const long dv = n / L, md = n % L, mul_c = dv * rf * L;
+			if (lane < L) {
+				uint64_t val = 0;
+				const long cnt = dv + (lane < md ? 1 : 0);
+				for (long pp = 0; pp < cnt; pp++) {
+					const uint8_t lb = lens[ks + pp * L + lane];
+					const int len = lb & 15, tz = (lb >> 4) << 2;
+					if (len) {
+						val += load_le(stream + cursor, len) << tz;
+						cursor += len;
+					}
+					double d;
+					memcpy(&d, &val, 8);
+					for (long t = 0; t < rf; t++) {
+						const long cpos = pp < dv ? (pp * rf + t) * L + lane : mul_c + lane * rf + t;
+						const uint64_t c = load_le(coords + (cs + cpos) * cb, cb);
+						const uint32_t rr = (uint32_t) (c & row_mask);
+						const uint32_t cc = (uint32_t) (c >> row_bits);
+						if (MODE == 0) {
+							atomicAdd(&yl[wv][rr], d * x[col_min + cc]);
+						} else {
+							const long o = nnz0 + cs + cpos;
+							rows_out[o] = (int32_t) (row_min + rr);
+							cols_out[o] = (int32_t) (col_min + cc);
+							vals_out[o] = d;
+						}
+					}
+				}
+			}
+			ks += n;
+			cs += n * rf;
+		}
+		if (MODE == 0) {
+			wave_fence();
+			for (uint32_t r = lane; r < nrows; r += kWave)
+				y[row_min + r] += yl[wv][r];
+			wave_fence();
+		}
+	}
 }
+
+__global__ void fill_zero(double *y, long m)
+{
+	long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < m)
+		y[i] = 0.0;
+}
+
+int div_convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col, const double *h_val)
+{
+	if (!h_rp)
+		return set_error(SPMVHIP_EINVAL, "div_rf: host CSR arrays are required");
+	const int T = f->p.div_threads > 0 ? f->p.div_threads : 1;
+	const int L = f->p.div_lanes > 0 ? f->p.div_lanes : 64;
+	const long P = f->p.div_packet_vals > 0 ? f->p.div_packet_vals : 16384;
+	if (L > kWave)
+		return set_error(SPMVHIP_EINVAL, "div_rf: %d lanes > the 64-lane wavefront", L);
+	if (T > 4096)
+		return set_error(SPMVHIP_EINVAL, "div_rf: %d partitions", T);
+	f->p.div_threads = T;
+	f->p.div_lanes = L;
+	f->p.div_packet_vals = P;
+	f->div_lanes = L;
+	spmvdiv::Stream st;
+	const char *err = nullptr;
+	if (spmvdiv::encode(h_rp, h_col, h_val, f->m, f->n, f->nnz, T, L, P, &st, &err))
+		return set_error(SPMVHIP_EFORMAT, "div_rf: %s (the reference rejects this matrix too)", err);
+	const long np = st.npackets(), ng = st.ngroups();
+	// 16-byte aligned packet starts on the device
+	std::vector<int64_t> doff(np + 1);
+	int64_t at = 0;
+	for (long p = 0; p < np; p++) {
+		doff[p] = at;
+		at += (st.pk_off[p + 1] - st.pk_off[p] + 15) / 16 * 16;
+	}
+	doff[np] = at;
+	std::vector<uint8_t> dev(at > 0 ? at : 16, 0);
+	for (long p = 0; p < np; p++)
+		memcpy(&dev[doff[p]], &st.bytes[st.pk_off[p]], st.pk_off[p + 1] - st.pk_off[p]);
+	f->div_npackets = np;
+	f->div_ngroups = ng;
+	f->div_bytes = (long) st.bytes.size();
+	SPMVHIP_TRY(f->div_stream.alloc(dev.size()));
+	SPMVHIP_TRY(f->div_pk_off.alloc((np + 1) * sizeof(int64_t)));
+	SPMVHIP_TRY(f->div_pk_nnz.alloc((np + 1) * sizeof(int64_t)));
+	SPMVHIP_TRY(f->div_groups.alloc((ng + 1) * sizeof(int32_t)));
+	SPMVHIP_TRY(f->div_group_rows.alloc((ng + 1) * sizeof(int32_t)));
+	hipStream_t s = f->stream;
+	SPMVHIP_CHECK(hipMemcpyAsync(f->div_stream.p, dev.data(), dev.size(), hipMemcpyHostToDevice, s));
+	SPMVHIP_CHECK(hipMemcpyAsync(f->div_pk_off.p, doff.data(), (np + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+	SPMVHIP_CHECK(hipMemcpyAsync(f->div_pk_nnz.p, st.pk_nnz.data(), (np + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+	SPMVHIP_CHECK(hipMemcpyAsync(f->div_groups.p, st.grp_pk.data(), (ng + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+	SPMVHIP_CHECK(hipMemcpyAsync(f->div_group_rows.p, st.grp_row.data(), (ng + 1) * sizeof(int32_t),
+			hipMemcpyHostToDevice, s));
+	SPMVHIP_CHECK(hipStreamSynchronize(s));
+	f->row_ptr.release();
+	f->col.release();
+	f->val.release();
+	f->mem_footprint = (double) f->div_bytes + (double) (np + 1) * 16 + (double) (ng + 1) * 8;
+	return SPMVHIP_OK;
+}
+
+int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
+{
+	if (f->m == 0)
+		return SPMVHIP_OK;
+	if (f->div_ngroups == 0) {
+		fill_zero<<<grid_for(f->m, kBlock), kBlock, 0, s>>>(y, f->m);
+		SPMVHIP_CHECK(hipGetLastError());
+		return SPMVHIP_OK;
+	}
+	div_kernel<0><<<grid_for(f->div_ngroups, kDivWavesPerBlock), kBlock, 0, s>>>(f->div_stream.as<uint8_t>(),
+			f->div_pk_off.as<int64_t>(), f->div_pk_nnz.as<int64_t>(), f->div_groups.as<int32_t>(),
+			f->div_group_rows.as<int32_t>(), f->div_ngroups, f->div_lanes, x, y, nullptr, nullptr, nullptr);
+	SPMVHIP_CHECK(hipGetLastError());
+	return SPMVHIP_OK;
+}
+
+int div_export_decoded(const spmvhip_format *f, int which, void *buf, size_t *bytes)
+{
+	size_t need = f->nnz * (which == 8 ? sizeof(double) : sizeof(int32_t));
+	if (!buf) {
+		*bytes = need;
+		return SPMVHIP_OK;
+	}
+	if (*bytes < need)
+		return set_error(SPMVHIP_EINVAL, "export %d needs %zu bytes", which, need);
+	if (f->nnz == 0) {
+		*bytes = 0;
+		return SPMVHIP_OK;
+	}
+	DevBuf rows, cols, vals;
+	SPMVHIP_TRY(rows.alloc(f->nnz * sizeof(int32_t)));
+	SPMVHIP_TRY(cols.alloc(f->nnz * sizeof(int32_t)));
+	SPMVHIP_TRY(vals.alloc(f->nnz * sizeof(double)));
+	div_kernel<1><<<grid_for(f->div_ngroups, kDivWavesPerBlock), kBlock, 0, f->stream>>>(f->div_stream.as<uint8_t>(),
+			f->div_pk_off.as<int64_t>(), f->div_pk_nnz.as<int64_t>(), f->div_groups.as<int32_t>(),
+			f->div_group_rows.as<int32_t>(), f->div_ngroups, f->div_lanes, nullptr, nullptr, rows.as<int32_t>(),
+			cols.as<int32_t>(), vals.as<double>());
+	SPMVHIP_CHECK(hipGetLastError());
+	SPMVHIP_CHECK(hipStreamSynchronize(f->stream));
+	const void *src = which == 6 ? rows.p : which == 7 ? cols.p : vals.p;
+	SPMVHIP_CHECK(hipMemcpy(buf, src, need, hipMemcpyDeviceToHost));
+	*bytes = need;
+	return SPMVHIP_OK;
+}
+
 } // namespace spmvhip
+
+extern "C" int spmvhip_div_encode(const int32_t *row_ptr, const int32_t *col_ind, const double *values, long m,
+		long n, long nnz, int T, int L, long P, uint8_t *bytes, size_t *nbytes, int64_t *pk_off, long *npackets,
+		int32_t *grp_pk, long *ngroups)
+{
+	using spmvhip::set_error;
+	if (!row_ptr || (nnz > 0 && (!col_ind || !values)) || !nbytes || !npackets || !ngroups)
+		return set_error(SPMVHIP_EINVAL, "NULL argument");
+	if (T < 1 || L < 1 || P < 1 || m < 0)
+		return set_error(SPMVHIP_EINVAL, "T, L, P must be positive");
+	spmvdiv::Stream st;
+	const char *err = nullptr;
+	if (spmvdiv::encode(row_ptr, col_ind, values, m, n, nnz, T, L, P, &st, &err))
+		return set_error(SPMVHIP_EFORMAT, "div_rf: %s", err);
+	if (bytes) {
+		if (*nbytes < st.bytes.size() || *npackets < st.npackets() || *ngroups < st.ngroups())
+			return set_error(SPMVHIP_EINVAL, "output buffers too small");
+		memcpy(bytes, st.bytes.data(), st.bytes.size());
+		if (pk_off)
+			memcpy(pk_off, st.pk_off.data(), st.pk_off.size() * sizeof(int64_t));
+		if (grp_pk)
+			memcpy(grp_pk, st.grp_pk.data(), st.grp_pk.size() * sizeof(int32_t));
+	}
+	*nbytes = st.bytes.size();
+	*npackets = st.npackets();
+	*ngroups = st.ngroups();
+	return SPMVHIP_OK;
+}

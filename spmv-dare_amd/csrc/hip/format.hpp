@@ -34,7 +34,8 @@ struct spmvhip_format {
 
 	// DIV_RF
 	spmvhip::DevBuf div_stream;   // packet bytes
-	spmvhip::DevBuf div_pk_off;   // int64 per packet: byte offset
+	spmvhip::DevBuf div_pk_off;   // int64 per packet: byte offset (16-byte aligned)
+	spmvhip::DevBuf div_pk_nnz;   // int64 per packet: first nnz index
 	spmvhip::DevBuf div_groups;   // int32 per packet group: first packet
 	spmvhip::DevBuf div_group_rows; // int32 per packet group: [row_begin,row_end)
 	long div_npackets = 0, div_ngroups = 0, div_bytes = 0;
@@ -50,6 +51,7 @@ struct spmvhip_format {
 	double conv_ms = 0, mem_footprint = 0, csr_mem_footprint = 0;
 	double stat_ms_sum = 0;
 	long stat_calls = 0;
+	long drv_calls = 0;   // spmvhip_spmv calls since statistics_start
 };
 
 namespace spmvhip {

@@ -68,6 +68,9 @@ _sig = {
                            ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "spmvhip_export": ([_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "spmvhip_destroy": ([_vp], None),
+    "spmvhip_div_encode": ([_vp, _vp, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_int,
+                            ctypes.c_int, ctypes.c_long, _vp, ctypes.POINTER(ctypes.c_size_t), _vp,
+                            ctypes.POINTER(ctypes.c_long), _vp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
 }
 for _name, (_args, _res) in _sig.items():
     _f = getattr(_lib, _name)
@@ -244,6 +247,26 @@ def csr_to_format_device(d_row_ptr, d_col_ind, d_values, m, n, nnz, kind="csr_ve
                                            ctypes.byref(p), ctypes.byref(h))
     _check(rc, "csr_to_format_device(%s)" % kind)
     return HipFormat(h.value, p)
+
+
+def div_encode(row_ptr, col_ind, values, n, T=1, L=64, P=16384):
+    """Host DIV-RF encoding in the reference's packet layout (no device
+    needed): returns (bytes, packet byte offsets, first packet of each group)."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    ci = np.ascontiguousarray(col_ind, dtype=np.int32)
+    va = np.ascontiguousarray(values, dtype=np.float64)
+    m, nnz = rp.size - 1, ci.size
+    nb, npk, ng = ctypes.c_size_t(), ctypes.c_long(), ctypes.c_long()
+    _check(_lib.spmvhip_div_encode(rp.ctypes.data, ci.ctypes.data, va.ctypes.data, m, n, nnz, T, L, P,
+                                   None, ctypes.byref(nb), None, ctypes.byref(npk), None, ctypes.byref(ng)),
+           "div_encode")
+    b = np.empty(max(nb.value, 1), dtype=np.uint8)
+    off = np.empty(npk.value + 1, dtype=np.int64)
+    grp = np.empty(ng.value + 1, dtype=np.int32)
+    _check(_lib.spmvhip_div_encode(rp.ctypes.data, ci.ctypes.data, va.ctypes.data, m, n, nnz, T, L, P,
+                                   b.ctypes.data, ctypes.byref(nb), off.ctypes.data, ctypes.byref(npk),
+                                   grp.ctypes.data, ctypes.byref(ng)), "div_encode")
+    return b[:nb.value], off, grp
 
 
 def statistics_print_labels():

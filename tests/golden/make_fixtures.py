@@ -165,6 +165,70 @@ def main():
         np.savez_compressed(os.path.join(HERE, "tiny_csr.npz"), row_ptr=rp, col_ind=ci, values=va)
     with open(os.path.join(HERE, "sellcs.json"), "w") as f:
         json.dump(sell, f, indent=1, sort_keys=True)
+    div_fixtures(gen)
+
+
+def read_div(path):
+    """Per partition: (stream bytes, packet group byte positions)."""
+    with open(path, "rb") as f:
+        T = struct.unpack("<q", f.read(8))[0]
+        parts = []
+        for _ in range(T):
+            nb = struct.unpack("<q", f.read(8))[0]
+            b = np.frombuffer(f.read(nb), dtype=np.uint8)
+            ng = struct.unpack("<q", f.read(8))[0]
+            pos = np.frombuffer(f.read(8 * (ng + 1)), dtype=np.int64)
+            parts.append((b, pos))
+    return parts
+
+
+def mask_padding(b, L=4):
+    """Zero the two padding bytes of every packet header (offsets 22-23),
+    which the reference leaves uninitialised."""
+    b = np.array(b, dtype=np.uint8, copy=True)
+    at = 0
+    while at < b.size:
+        h = b[at:at + 28 + 4 * L].view(np.uint32)
+        nv, nu, nrfs = int(h[0]), int(h[1]), int(h[6])
+        cb = (int(b[at + 20]) + int(b[at + 21]) + 7) >> 3
+        size = 28 + 4 * L + 8 * nrfs + nv * cb + nu + int(h[7:7 + L].astype(np.int64).sum())
+        b[at + 22:at + 24] = 0
+        at += size
+    return b
+
+
+def div_fixtures(gen):
+    """DIV-RF streams of the reference (AVX2 build: 4 value lanes) at several
+    partition counts T (OMP threads) and packet sizes P."""
+    if not os.path.exists(os.path.join(REF, "div_dump")):
+        sys.exit("oracle/_ref/div_dump missing: run `make -C oracle ref` first")
+    cases = [("test_txt", 1, 16384), ("test_txt", 4, 512), ("test_txt", 3, 97),
+             ("cfg1", 1, 16384), ("cfg1", 8, 4096), ("neigh_two", 2, 16384),
+             ("batch_03", 5, 1000), ("dense_rows", 1, 2048), ("tall", 4, 16384),
+             ("simple_placement", 2, 300), ("degree_clamp", 3, 16384)]
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        for name, T, P in cases:
+            line = gen[name]["line"]
+            csr = os.path.join(td, "c.bin")
+            run_gen(line, csr)
+            dump = os.path.join(td, "d.bin")
+            env = dict(os.environ, OMP_NUM_THREADS=str(T))
+            subprocess.run([os.path.join(REF, "div_dump"), csr, str(P), dump], check=True,
+                           capture_output=True, env=env)
+            parts = read_div(dump)
+            stream = np.concatenate([mask_padding(b) for b, _ in parts]) if parts else np.zeros(0, np.uint8)
+            groups = [int(p.size - 1) for _, p in parts]
+            key = "%s/T%d/P%d" % (name, T, P)
+            out[key] = {"line": line, "T": T, "L": 4, "P": P, "bytes": int(stream.size),
+                        "stream": digest(stream), "groups_per_partition": groups,
+                        "group_pos": digest(np.concatenate([p for _, p in parts]))}
+            print(key, stream.size, flush=True)
+            if key == "test_txt/T4/P512":
+                np.savez_compressed(os.path.join(HERE, "tiny_div.npz"), stream=stream,
+                                    **{"group_pos_%d" % t: p for t, (_, p) in enumerate(parts)})
+    with open(os.path.join(HERE, "div.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
