@@ -162,6 +162,52 @@ int spmvhip_export(const spmvhip_format *f, int which, void *buf, size_t *bytes_
 
 void spmvhip_destroy(spmvhip_format *f);
 
+/* ---- batched SpMV (no counterpart in the reference, whose driver runs one
+ * matrix per process, run.sh:116-151) ----
+ * Many converted matrices, each with its own device x and y, computed by ONE
+ * persistent kernel launch that draws (matrix, unit range) work items from an
+ * atomic ticket; with use_graph the launch (ticket reset + kernel) is
+ * captured once into a hipGraph and replayed. Members must be CSR_VEC
+ * (block_nnz <= 512), SELL or SELL_C_S handles, and must outlive the batch.
+ * tile_units: row blocks / slices per work item (0 = 64). */
+typedef struct spmvhip_batch spmvhip_batch;
+int spmvhip_batch_create(int tile_units, spmvhip_batch **out);
+int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d_x, double *d_y);
+int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph);
+int spmvhip_batch_run(spmvhip_batch *b);          /* asynchronous on the batch's stream */
+int spmvhip_batch_synchronize(spmvhip_batch *b);
+/* ms[i]: time of iteration i of the batched launch (per_matrix = 0) or of
+ * one standalone spmv launch per member, back to back (per_matrix = 1). */
+int spmvhip_batch_time(spmvhip_batch *b, int per_matrix, int warmup, int iters, float *ms);
+int spmvhip_batch_info(const spmvhip_batch *b, long *nmats, long *nitems, int *grid, long *total_nnz);
+void spmvhip_batch_destroy(spmvhip_batch *b);
+
+/* ---- format selector support (no counterpart in the reference) ---- */
+
+/* Structure statistics of a device CSR matrix, the selector's cheap
+ * features, with the definitions of the reference generator's descriptor
+ * statistics (artificial_matrix_generation.c:503-539, matrix_util.h:235-266):
+ * out6 = {mean row length, std of row lengths, longest row, mean bandwidth / n,
+ * neighbours per nnz (window 1), non-empty rows}. */
+int spmvhip_csr_stats_device(const int32_t *d_row_ptr, const int32_t *d_col_ind, long m, long n,
+		double *out6);
+
+/* FP32 MLP n_in -> h1 -> h2 -> n_out (ReLU hidden, linear out) with
+ * hand-written forward, backward (masked mean squared error) and Adam
+ * kernels. Parameters: one flat array, per layer W[n_in][n_out] then b. */
+typedef struct spmvhip_mlp spmvhip_mlp;
+int spmvhip_mlp_create(int n_in, int h1, int h2, int n_out, unsigned int seed, spmvhip_mlp **out);
+long spmvhip_mlp_num_params(const spmvhip_mlp *net);
+int spmvhip_mlp_set_params(spmvhip_mlp *net, const float *host, long count);
+int spmvhip_mlp_get_params(const spmvhip_mlp *net, float *host, long count);
+int spmvhip_mlp_predict(spmvhip_mlp *net, const float *X, int batch, float *Y);
+int spmvhip_mlp_gradients(spmvhip_mlp *net, const float *X, const float *T, const float *M, int batch,
+		float *grads, float *loss);
+int spmvhip_mlp_train(spmvhip_mlp *net, const float *X, const float *T, const float *M, long rows,
+		int batch, int epochs, float lr, float *epoch_losses);
+int spmvhip_mlp_adam_step(spmvhip_mlp *net, const float *grads, float lr);
+void spmvhip_mlp_destroy(spmvhip_mlp *net);
+
 /* The host half of csr_to_format for DIV_RF, exposed so that the encoder can
  * be checked without a GPU: encodes the CSR matrix into the reference's
  * packet layout with T nnz-balanced partitions, L value lanes and packets

@@ -1,0 +1,301 @@
+// Batched SpMV: many independent converted matrices, each with its own x and
+// y, computed by ONE persistent kernel launch that takes work from a queue
+// (an atomic ticket over (matrix, unit range) items), optionally replayed
+// from a captured hipGraph. The reference has no counterpart: its driver runs
+// one matrix per process (run.sh:116-151).
+//
+// Units are the formats' own per-wave work: a row block of the binned CSR
+// format or a 64-row slice of SELL / SELL-C-sigma (kernels.hpp). A work item
+// is `tile` consecutive units of one matrix; a workgroup (4 waves) takes one
+// item per ticket, its waves striding over the item's units. Long CSR rows
+// are summed by one wave (the split path of the standalone kernel is not
+// used here), so any row length is handled in the single launch.
+
+#include <algorithm>
+#include <vector>
+
+#include "format.hpp"
+#include "kernels.hpp"
+
+namespace spmvhip {
+
+constexpr int kBatchBnnz = 512; // LDS staging per wave of CSR blocks
+
+struct BatchMat {
+	int kind, lanes;
+	long m;
+	const int32_t *rp, *col, *blocks;
+	const double *val;
+	const int64_t *sptr;
+	const int32_t *sw, *scol, *order;
+	const double *sval;
+	const double *x;
+	double *y;
+};
+
+struct WorkItem {
+	int mat, u0, u1, pad;
+};
+
+__global__ void batch_reset(int *counter)
+{
+	if (threadIdx.x == 0)
+		*counter = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) batch_kernel(const BatchMat *__restrict__ mats,
+		const WorkItem *__restrict__ items, int nitems, int *__restrict__ counter)
+{
+	__shared__ double prod[kWavesPerBlock][2 * kBatchBnnz];
+	__shared__ int ticket;
+	const int wv = threadIdx.x / kWave;
+	for (;;) {
+		if (threadIdx.x == 0)
+			ticket = atomicAdd(counter, 1);
+		__syncthreads();
+		const int it = ticket;
+		__syncthreads(); // every wave has read the ticket before the next one is drawn
+		if (it >= nitems)
+			break; // uniform across the workgroup: every wave leaves together
+		const WorkItem wi = items[it];
+		const BatchMat &M = mats[wi.mat];
+		if (M.kind == SPMVHIP_CSR_VEC) {
+			for (int u = wi.u0 + wv; u < wi.u1; u += kWavesPerBlock)
+				csr_block<kBatchBnnz>(M.rp, M.col, M.val, M.x, M.y, M.blocks[u], M.blocks[u + 1], 0, false,
+						M.lanes, prod[wv]);
+		} else {
+			for (int u = wi.u0 + wv; u < wi.u1; u += kWavesPerBlock)
+				sell_slice<4>(M.sptr, M.sw, M.scol, M.sval, M.order, M.x, M.y, M.m, u);
+		}
+	}
+}
+
+} // namespace spmvhip
+
+struct spmvhip_batch {
+	int tile = 64;
+	std::vector<spmvhip_format *> fmts;
+	std::vector<spmvhip::BatchMat> host_mats;
+	std::vector<spmvhip::WorkItem> host_items;
+	spmvhip::DevBuf mats, items, counter;
+	hipStream_t stream = nullptr;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	int grid = 0;
+	bool finalized = false;
+	long total_nnz = 0;
+};
+
+using namespace spmvhip;
+
+namespace {
+
+int batch_launch(spmvhip_batch *b, hipStream_t s)
+{
+	batch_reset<<<1, kWave, 0, s>>>(b->counter.as<int>());
+	SPMVHIP_CHECK(hipGetLastError());
+	if (!b->host_items.empty()) {
+		batch_kernel<<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->items.as<WorkItem>(),
+				(int) b->host_items.size(), b->counter.as<int>());
+		SPMVHIP_CHECK(hipGetLastError());
+	}
+	return SPMVHIP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int spmvhip_batch_create(int tile_units, spmvhip_batch **out)
+{
+	if (!out)
+		return set_error(SPMVHIP_EINVAL, "out is NULL");
+	*out = nullptr;
+	if (tile_units < 0 || tile_units > 65536)
+		return set_error(SPMVHIP_EINVAL, "tile_units %d", tile_units);
+	spmvhip_batch *b = new (std::nothrow) spmvhip_batch();
+	if (!b)
+		return set_error(SPMVHIP_ENOMEM, "batch handle");
+	b->tile = tile_units > 0 ? tile_units : 64;
+	hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+	if (e != hipSuccess) {
+		delete b;
+		return set_error(SPMVHIP_EHIP, "stream: %s", hipGetErrorString(e));
+	}
+	*out = b;
+	return SPMVHIP_OK;
+}
+
+int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d_x, double *d_y)
+{
+	if (!b || !f || !d_x || !d_y)
+		return set_error(SPMVHIP_EINVAL, "NULL argument");
+	if (b->finalized)
+		return set_error(SPMVHIP_EINVAL, "batch already finalized");
+	BatchMat M{};
+	M.kind = f->kind;
+	M.m = f->m;
+	M.x = d_x;
+	M.y = d_y;
+	long units;
+	if (f->kind == SPMVHIP_CSR_VEC) {
+		if (f->block_nnz > kBatchBnnz)
+			return set_error(SPMVHIP_EINVAL, "batch: csr_vec block_nnz %d > %d", f->block_nnz, kBatchBnnz);
+		M.rp = f->row_ptr.as<int32_t>();
+		M.col = f->col.as<int32_t>();
+		M.val = f->val.as<double>();
+		M.blocks = f->blocks.as<int32_t>();
+		M.lanes = f->lanes;
+		units = f->nblocks;
+	} else if (f->kind == SPMVHIP_SELL || f->kind == SPMVHIP_SELL_C_S) {
+		M.sptr = f->slice_ptr.as<int64_t>();
+		M.sw = f->slice_w.as<int32_t>();
+		M.scol = f->s_cols.as<int32_t>();
+		M.sval = f->s_vals.as<double>();
+		M.order = f->kind == SPMVHIP_SELL_C_S ? f->row_order.as<int32_t>() : nullptr;
+		units = f->nslices;
+	} else {
+		return set_error(SPMVHIP_EINVAL, "batch: format %s is not batchable (csr_vec, sell, sell_c_s are)",
+				f->name);
+	}
+	if (units > INT32_MAX)
+		return set_error(SPMVHIP_EINVAL, "batch: too many units");
+	int mi = (int) b->host_mats.size();
+	b->host_mats.push_back(M);
+	b->fmts.push_back(const_cast<spmvhip_format *>(f));
+	for (long u = 0; u < units; u += b->tile)
+		b->host_items.push_back({mi, (int) u, (int) std::min(units, u + b->tile), 0});
+	b->total_nnz += f->nnz;
+	return SPMVHIP_OK;
+}
+
+int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
+{
+	if (!b)
+		return set_error(SPMVHIP_EINVAL, "NULL batch");
+	if (b->finalized)
+		return set_error(SPMVHIP_EINVAL, "batch already finalized");
+	if (b->host_items.size() > (size_t) INT32_MAX)
+		return set_error(SPMVHIP_EINVAL, "batch: too many work items");
+	SPMVHIP_TRY(b->mats.alloc(std::max<size_t>(1, b->host_mats.size()) * sizeof(BatchMat)));
+	SPMVHIP_TRY(b->items.alloc(std::max<size_t>(1, b->host_items.size()) * sizeof(WorkItem)));
+	SPMVHIP_TRY(b->counter.alloc(sizeof(int)));
+	if (!b->host_mats.empty())
+		SPMVHIP_CHECK(hipMemcpy(b->mats.p, b->host_mats.data(), b->host_mats.size() * sizeof(BatchMat),
+				hipMemcpyHostToDevice));
+	if (!b->host_items.empty())
+		SPMVHIP_CHECK(hipMemcpy(b->items.p, b->host_items.data(), b->host_items.size() * sizeof(WorkItem),
+				hipMemcpyHostToDevice));
+	// persistent grid: as many workgroups as can be resident at once
+	int dev = 0, per_cu = 0;
+	hipDeviceProp_t prop;
+	SPMVHIP_CHECK(hipGetDevice(&dev));
+	SPMVHIP_CHECK(hipGetDeviceProperties(&prop, dev));
+	SPMVHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel, kBlock, 0));
+	long g = (long) prop.multiProcessorCount * std::max(per_cu, 1);
+	b->grid = (int) std::max<long>(1, std::min<long>(g, (long) b->host_items.size()));
+	b->finalized = true;
+	if (use_graph) {
+		SPMVHIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+		int rc = batch_launch(b, b->stream);
+		hipGraph_t graph = nullptr;
+		hipError_t e = hipStreamEndCapture(b->stream, &graph);
+		if (rc)
+			return rc;
+		if (e != hipSuccess)
+			return set_error(SPMVHIP_EHIP, "graph capture: %s", hipGetErrorString(e));
+		b->graph = graph;
+		SPMVHIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+	}
+	return SPMVHIP_OK;
+}
+
+int spmvhip_batch_run(spmvhip_batch *b)
+{
+	if (!b || !b->finalized)
+		return set_error(SPMVHIP_EINVAL, "batch not finalized");
+	if (b->exec) {
+		SPMVHIP_CHECK(hipGraphLaunch(b->exec, b->stream));
+		return SPMVHIP_OK;
+	}
+	return batch_launch(b, b->stream);
+}
+
+int spmvhip_batch_synchronize(spmvhip_batch *b)
+{
+	if (!b)
+		return set_error(SPMVHIP_EINVAL, "NULL batch");
+	SPMVHIP_CHECK(hipStreamSynchronize(b->stream));
+	return SPMVHIP_OK;
+}
+
+int spmvhip_batch_time(spmvhip_batch *b, int per_matrix, int warmup, int iters, float *ms)
+{
+	if (!b || !b->finalized || iters < 1 || warmup < 0)
+		return set_error(SPMVHIP_EINVAL, "bad arguments");
+	auto once = [&]() -> int {
+		if (!per_matrix)
+			return spmvhip_batch_run(b);
+		for (size_t i = 0; i < b->fmts.size(); i++)
+			SPMVHIP_TRY(spmvhip_spmv_device(b->fmts[i], b->host_mats[i].x, b->host_mats[i].y, b->stream));
+		return SPMVHIP_OK;
+	};
+	for (int i = 0; i < warmup; i++)
+		SPMVHIP_TRY(once());
+	hipEvent_t e0, e1;
+	SPMVHIP_CHECK(hipEventCreate(&e0));
+	SPMVHIP_CHECK(hipEventCreate(&e1));
+	int rc = SPMVHIP_OK;
+	for (int i = 0; i < iters && rc == SPMVHIP_OK; i++) {
+		hipError_t he = hipEventRecord(e0, b->stream);
+		if (he == hipSuccess)
+			rc = once();
+		if (he == hipSuccess && rc == SPMVHIP_OK)
+			he = hipEventRecord(e1, b->stream);
+		if (he == hipSuccess && rc == SPMVHIP_OK)
+			he = hipEventSynchronize(e1);
+		float t = 0;
+		if (he == hipSuccess && rc == SPMVHIP_OK)
+			he = hipEventElapsedTime(&t, e0, e1);
+		if (he != hipSuccess && rc == SPMVHIP_OK)
+			rc = set_error(SPMVHIP_EHIP, "batch timing: %s", hipGetErrorString(he));
+		if (ms)
+			ms[i] = t;
+	}
+	(void) hipEventDestroy(e0);
+	(void) hipEventDestroy(e1);
+	return rc;
+}
+
+int spmvhip_batch_info(const spmvhip_batch *b, long *nmats, long *nitems, int *grid, long *total_nnz)
+{
+	if (!b)
+		return set_error(SPMVHIP_EINVAL, "NULL batch");
+	if (nmats)
+		*nmats = (long) b->host_mats.size();
+	if (nitems)
+		*nitems = (long) b->host_items.size();
+	if (grid)
+		*grid = b->grid;
+	if (total_nnz)
+		*total_nnz = b->total_nnz;
+	return SPMVHIP_OK;
+}
+
+void spmvhip_batch_destroy(spmvhip_batch *b)
+{
+	if (!b)
+		return;
+	if (b->stream)
+		(void) hipStreamSynchronize(b->stream);
+	if (b->exec)
+		(void) hipGraphExecDestroy(b->exec);
+	if (b->graph)
+		(void) hipGraphDestroy(b->graph);
+	hipStream_t s = b->stream;
+	delete b;
+	if (s)
+		(void) hipStreamDestroy(s);
+}
+
+} // extern "C"

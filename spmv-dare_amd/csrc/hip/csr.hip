@@ -19,6 +19,7 @@
 //     segment order (several waves per row, deterministic y).
 
 #include "format.hpp"
+#include "kernels.hpp"
 #include "scan.hpp"
 
 namespace spmvhip {
@@ -211,58 +212,11 @@ __global__ void __launch_bounds__(kBlock) csr_binned_kernel(const int32_t *__res
 		const int32_t *__restrict__ blocks, long nblocks, int split_nnz, int lanes_cfg, int remap)
 {
 	__shared__ double prod[kWavesPerBlock][2 * BNNZ_MAX];
-	const int lane = lane_id(), wv = threadIdx.x / kWave;
+	const int wv = threadIdx.x / kWave;
 	const long w = xcd_logical_block(remap) * kWavesPerBlock + wv;
 	if (w >= nblocks)
 		return; // waves never synchronise with each other: early exit is safe
-	const int r0 = blocks[w], r1 = blocks[w + 1];
-	const int j0 = rp[r0], j1 = rp[r1];
-	const int nrows = r1 - r0, nz = j1 - j0;
-
-	if (nrows == 1 && nz > BNNZ_MAX) {
-		if (nz > split_nnz)
-			return; // split row: handled by csr_split_kernel
-		// wave per row
-		double a0 = 0, a1 = 0;
-		int j = j0 + lane;
-		for (; j + kWave < j1; j += 2 * kWave) {
-			a0 = fma(val[j], x[col[j]], a0);
-			a1 = fma(val[j + kWave], x[col[j + kWave]], a1);
-		}
-		if (j < j1)
-			a0 = fma(val[j], x[col[j]], a0);
-		double sum = wave_sum(a0 + a1);
-		if (lane == 0)
-			y[r0] = sum;
-		return;
-	}
-
-	// stream block: products to LDS, coalesced over the block's nnz
-	double *p = prod[wv];
-	for (int e = lane; e < nz; e += kWave)
-		p[e] = val[j0 + e] * x[col[j0 + e]];
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-	// lanes per row: largest power of two <= 64 / nrows (or the configured value)
-	int L = kWave;
-	while (L > 1 && L * nrows > kWave)
-		L >>= 1;
-	if (lanes_cfg > 0 && lanes_cfg < L)
-		L = lanes_cfg;
-	const int g = lane / L, sub = lane - g * L, ngroups = kWave / L;
-	for (int r = g; r < nrows + ((ngroups - (nrows % ngroups)) % ngroups); r += ngroups) {
-		double s = 0;
-		if (r < nrows) {
-			const int a = rp[r0 + r] - j0, b = rp[r0 + r + 1] - j0;
-			for (int e = a + sub; e < b; e += L)
-				s += p[e];
-		}
-		s = group_sum(s, L);
-		if (r < nrows && sub == 0)
-			y[r0 + r] = s;
-	}
+	csr_block<BNNZ_MAX>(rp, col, val, x, y, blocks[w], blocks[w + 1], split_nnz, true, lanes_cfg, prod[wv]);
 }
 
 __global__ void __launch_bounds__(kBlock) csr_split_kernel(const int32_t *__restrict__ col, const double *__restrict__ val,
@@ -272,16 +226,7 @@ __global__ void __launch_bounds__(kBlock) csr_split_kernel(const int32_t *__rest
 	const long w = (long) blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
 	if (w >= nsegs)
 		return;
-	const int j0 = desc[3 * w + 1], j1 = desc[3 * w + 2];
-	double a0 = 0, a1 = 0;
-	int j = j0 + lane;
-	for (; j + kWave < j1; j += 2 * kWave) {
-		a0 = fma(val[j], x[col[j]], a0);
-		a1 = fma(val[j + kWave], x[col[j + kWave]], a1);
-	}
-	if (j < j1)
-		a0 = fma(val[j], x[col[j]], a0);
-	double sum = wave_sum(a0 + a1);
+	const double sum = wave_row_sum(col, val, x, desc[3 * w + 1], desc[3 * w + 2]);
 	if (lane == 0)
 		partials[w] = sum;
 }

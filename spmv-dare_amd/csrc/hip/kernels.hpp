@@ -1,0 +1,114 @@
+// Per-wavefront SpMV work units shared by the standalone kernels (csr.hip,
+// sell.hip) and the batched persistent kernel (batch.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace spmvhip {
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// y[r] = sum of row r over [j0, j1), one wave, lanes striding the row.
+__device__ __forceinline__ double wave_row_sum(const int32_t *__restrict__ col, const double *__restrict__ val,
+		const double *__restrict__ x, int j0, int j1)
+{
+	const int lane = lane_id();
+	double a0 = 0, a1 = 0;
+	int j = j0 + lane;
+	for (; j + kWave < j1; j += 2 * kWave) {
+		a0 = fma(val[j], x[col[j]], a0);
+		a1 = fma(val[j + kWave], x[col[j + kWave]], a1);
+	}
+	if (j < j1)
+		a0 = fma(val[j], x[col[j]], a0);
+	return wave_sum(a0 + a1);
+}
+
+// One row block [r0, r1) of the binned CSR format (see csr.hip), processed
+// by the calling wavefront. `p` is the wave's LDS buffer of 2 * BNNZ_MAX
+// doubles. A single row longer than BNNZ_MAX is summed by the whole wave;
+// with skip_split, rows longer than split_nnz are left to the split kernels.
+template <int BNNZ_MAX>
+__device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+		const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y, int r0, int r1,
+		int split_nnz, bool skip_split, int lanes_cfg, double *p)
+{
+	const int lane = lane_id();
+	const int j0 = rp[r0], j1 = rp[r1];
+	const int nrows = r1 - r0, nz = j1 - j0;
+	if (nrows == 1 && nz > BNNZ_MAX) {
+		if (skip_split && nz > split_nnz)
+			return;
+		double sum = wave_row_sum(col, val, x, j0, j1);
+		if (lane == 0)
+			y[r0] = sum;
+		return;
+	}
+	// stream block: products to LDS, coalesced over the block's nnz
+	for (int e = lane; e < nz; e += kWave)
+		p[e] = val[j0 + e] * x[col[j0 + e]];
+	wave_lds_sync();
+	// lanes per row: largest power of two <= 64 / nrows (or the configured value)
+	int L = kWave;
+	while (L > 1 && L * nrows > kWave)
+		L >>= 1;
+	if (lanes_cfg > 0 && lanes_cfg < L)
+		L = lanes_cfg;
+	const int g = lane / L, sub = lane - g * L, ngroups = kWave / L;
+	for (int r = g; r < nrows + ((ngroups - (nrows % ngroups)) % ngroups); r += ngroups) {
+		double s = 0;
+		if (r < nrows) {
+			const int a = rp[r0 + r] - j0, b = rp[r0 + r + 1] - j0;
+			for (int e = a + sub; e < b; e += L)
+				s += p[e];
+		}
+		s = group_sum(s, L);
+		if (r < nrows && sub == 0)
+			y[r0 + r] = s;
+	}
+	wave_lds_sync(); // the buffer is reused by the wave's next block
+}
+
+// One SELL slice of 64 rows, lane = row: one fused multiply-add per stored
+// entry in entry order; UNROLL entries are loaded ahead of their FMAs.
+template <int UNROLL>
+__device__ __forceinline__ void sell_slice(const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
+		const int32_t *__restrict__ scol, const double *__restrict__ sval, const int32_t *__restrict__ order,
+		const double *__restrict__ x, double *__restrict__ y, long m, long s)
+{
+	constexpr int C = kWave;
+	const int lane = lane_id();
+	const int w = width[s];
+	const double *v = sval + sptr[s] + lane;
+	const int32_t *c = scol + sptr[s] + lane;
+	double acc = 0;
+	int k = 0;
+	for (; k + UNROLL <= w; k += UNROLL) {
+		double vv[UNROLL];
+		int32_t cc[UNROLL];
+#pragma unroll
+		for (int u = 0; u < UNROLL; u++) {
+			vv[u] = __builtin_nontemporal_load(v + (long) (k + u) * C);
+			cc[u] = __builtin_nontemporal_load(c + (long) (k + u) * C);
+		}
+		double xx[UNROLL];
+#pragma unroll
+		for (int u = 0; u < UNROLL; u++)
+			xx[u] = x[cc[u]];
+#pragma unroll
+		for (int u = 0; u < UNROLL; u++)
+			acc = fma(vv[u], xx[u], acc);
+	}
+	for (; k < w; k++)
+		acc = fma(v[(long) k * C], x[c[(long) k * C]], acc);
+	const long r = s * C + lane;
+	if (r < m)
+		y[order ? order[r] : r] = acc;
+}
+
+} // namespace spmvhip

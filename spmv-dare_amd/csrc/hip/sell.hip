@@ -29,6 +29,7 @@
 // rows < m are written.
 
 #include "format.hpp"
+#include "kernels.hpp"
 #include "scan.hpp"
 
 namespace spmvhip {
@@ -271,33 +272,7 @@ __global__ void __launch_bounds__(kBlock) sell_kernel(const int64_t *__restrict_
 	const long s = blk * kWavesPerBlock + threadIdx.x / kWave;
 	if (s >= nslices)
 		return;
-	const int lane = lane_id();
-	const int w = width[s];
-	const double *v = sval + sptr[s] + lane;
-	const int32_t *c = scol + sptr[s] + lane;
-	double acc = 0;
-	int k = 0;
-	for (; k + UNROLL <= w; k += UNROLL) {
-		double vv[UNROLL];
-		int32_t cc[UNROLL];
-#pragma unroll
-		for (int u = 0; u < UNROLL; u++) {
-			vv[u] = __builtin_nontemporal_load(v + (long) (k + u) * kSliceC);
-			cc[u] = __builtin_nontemporal_load(c + (long) (k + u) * kSliceC);
-		}
-		double xx[UNROLL];
-#pragma unroll
-		for (int u = 0; u < UNROLL; u++)
-			xx[u] = x[cc[u]];
-#pragma unroll
-		for (int u = 0; u < UNROLL; u++)
-			acc = fma(vv[u], xx[u], acc);
-	}
-	for (; k < w; k++)
-		acc = fma(v[(long) k * kSliceC], x[c[(long) k * kSliceC]], acc);
-	const long r = s * kSliceC + lane;
-	if (r < m)
-		y[order ? order[r] : r] = acc;
+	sell_slice<UNROLL>(sptr, width, scol, sval, order, x, y, m, s);
 }
 
 int sell_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)

@@ -68,6 +68,27 @@ _sig = {
                            ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "spmvhip_export": ([_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "spmvhip_destroy": ([_vp], None),
+    "spmvhip_batch_create": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "spmvhip_batch_add": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "spmvhip_batch_finalize": ([_vp, ctypes.c_int], ctypes.c_int),
+    "spmvhip_batch_run": ([_vp], ctypes.c_int),
+    "spmvhip_batch_synchronize": ([_vp], ctypes.c_int),
+    "spmvhip_batch_time": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "spmvhip_batch_info": ([_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),
+                            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+    "spmvhip_batch_destroy": ([_vp], None),
+    "spmvhip_csr_stats_device": ([_vp, _vp, ctypes.c_long, ctypes.c_long, _vp], ctypes.c_int),
+    "spmvhip_mlp_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint,
+                            ctypes.POINTER(_vp)], ctypes.c_int),
+    "spmvhip_mlp_num_params": ([_vp], ctypes.c_long),
+    "spmvhip_mlp_set_params": ([_vp, _vp, ctypes.c_long], ctypes.c_int),
+    "spmvhip_mlp_get_params": ([_vp, _vp, ctypes.c_long], ctypes.c_int),
+    "spmvhip_mlp_predict": ([_vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+    "spmvhip_mlp_gradients": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "spmvhip_mlp_train": ([_vp, _vp, _vp, _vp, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                           _vp], ctypes.c_int),
+    "spmvhip_mlp_adam_step": ([_vp, _vp, ctypes.c_float], ctypes.c_int),
+    "spmvhip_mlp_destroy": ([_vp], None),
     "spmvhip_div_encode": ([_vp, _vp, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_int,
                             ctypes.c_int, ctypes.c_long, _vp, ctypes.POINTER(ctypes.c_size_t), _vp,
                             ctypes.POINTER(ctypes.c_long), _vp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
@@ -247,6 +268,125 @@ def csr_to_format_device(d_row_ptr, d_col_ind, d_values, m, n, nnz, kind="csr_ve
                                            ctypes.byref(p), ctypes.byref(h))
     _check(rc, "csr_to_format_device(%s)" % kind)
     return HipFormat(h.value, p)
+
+
+class Batch:
+    """Many converted matrices computed by one persistent, work-queue
+    scheduled launch (optionally a replayed hipGraph). The member formats and
+    their device vectors must stay alive while the batch is used."""
+
+    def __init__(self, tile_units=0):
+        h = _vp()
+        _check(_lib.spmvhip_batch_create(tile_units, ctypes.byref(h)), "batch_create")
+        self._h = h.value
+        self._keep = []
+
+    def add(self, fmt, dx, dy):
+        _check(_lib.spmvhip_batch_add(self._h, fmt._h, dx.ptr, dy.ptr), "batch_add")
+        self._keep.append((fmt, dx, dy))
+
+    def finalize(self, use_graph=True):
+        _check(_lib.spmvhip_batch_finalize(self._h, 1 if use_graph else 0), "batch_finalize")
+
+    def run(self):
+        _check(_lib.spmvhip_batch_run(self._h), "batch_run")
+
+    def synchronize(self):
+        _check(_lib.spmvhip_batch_synchronize(self._h), "batch_synchronize")
+
+    def time(self, warmup, iters, per_matrix=False):
+        ms = (ctypes.c_float * iters)()
+        _check(_lib.spmvhip_batch_time(self._h, 1 if per_matrix else 0, warmup, iters, ms), "batch_time")
+        return np.array(ms[:], dtype=np.float64)
+
+    def info(self):
+        a, b, c, d = ctypes.c_long(), ctypes.c_long(), ctypes.c_int(), ctypes.c_long()
+        _check(_lib.spmvhip_batch_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                       ctypes.byref(d)), "batch_info")
+        return {"matrices": a.value, "items": b.value, "grid": c.value, "nnz": d.value}
+
+    def destroy(self):
+        if self._h:
+            _lib.spmvhip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def csr_stats_device(d_row_ptr, d_col_ind, m, n):
+    """Device structure statistics: mean / std / max row length, mean
+    bandwidth / n, neighbours per nnz, non-empty rows."""
+    out = np.empty(6, dtype=np.float64)
+    _check(_lib.spmvhip_csr_stats_device(d_row_ptr.ptr, d_col_ind.ptr, m, n, out.ctypes.data), "csr_stats")
+    keys = ("avg_nnz_per_row", "std_nnz_per_row", "max_nnz_per_row", "avg_bw_scaled",
+            "avg_num_neighbours", "nonempty_rows")
+    return dict(zip(keys, out.tolist()))
+
+
+class MLP:
+    """FP32 MLP with hand-written HIP forward / backward / Adam kernels."""
+
+    def __init__(self, n_in, h1, h2, n_out, seed=0):
+        h = _vp()
+        _check(_lib.spmvhip_mlp_create(n_in, h1, h2, n_out, seed, ctypes.byref(h)), "mlp_create")
+        self._h = h.value
+        self.dims = (n_in, h1, h2, n_out)
+        self.nparams = _lib.spmvhip_mlp_num_params(self._h)
+
+    def get_params(self):
+        p = np.empty(self.nparams, dtype=np.float32)
+        _check(_lib.spmvhip_mlp_get_params(self._h, p.ctypes.data, self.nparams), "mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(_lib.spmvhip_mlp_set_params(self._h, p.ctypes.data, p.size), "mlp_set_params")
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.empty((X.shape[0], self.dims[3]), dtype=np.float32)
+        _check(_lib.spmvhip_mlp_predict(self._h, X.ctypes.data, X.shape[0], Y.ctypes.data), "mlp_predict")
+        return Y
+
+    def gradients(self, X, T, M=None):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        Mp = None if M is None else np.ascontiguousarray(M, dtype=np.float32)
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(_lib.spmvhip_mlp_gradients(self._h, X.ctypes.data, T.ctypes.data,
+                                          None if Mp is None else Mp.ctypes.data, X.shape[0],
+                                          g.ctypes.data, ctypes.byref(loss)), "mlp_gradients")
+        return g, loss.value
+
+    def train(self, X, T, M=None, batch=128, epochs=1, lr=1e-3):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        Mp = None if M is None else np.ascontiguousarray(M, dtype=np.float32)
+        losses = np.empty(max(epochs, 1), dtype=np.float32)
+        _check(_lib.spmvhip_mlp_train(self._h, X.ctypes.data, T.ctypes.data,
+                                      None if Mp is None else Mp.ctypes.data, X.shape[0], batch, epochs,
+                                      lr, losses.ctypes.data), "mlp_train")
+        return losses[:epochs]
+
+    def adam_step(self, grads, lr):
+        g = np.ascontiguousarray(grads, dtype=np.float32)
+        _check(_lib.spmvhip_mlp_adam_step(self._h, g.ctypes.data, lr), "mlp_adam_step")
+
+    def destroy(self):
+        if self._h:
+            _lib.spmvhip_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def div_encode(row_ptr, col_ind, values, n, T=1, L=64, P=16384):

@@ -1,0 +1,188 @@
+"""Autotuner and learned format selector.
+
+Variant space: 40 format x tuning choices over the batchable drop-in formats
+(row-length-binned CSR with rows-per-wave / lanes / XCD mapping, SELL with
+its inner unroll, SELL-C-sigma with sigma and unroll). The exhaustive
+autotuner converts a matrix to every variant on the device and times it; the
+selector is an FP32 MLP 14 -> 64 -> 64 -> 40 (hand-written HIP kernels,
+spmvhip_mlp_*) that predicts every variant's slowdown log(t / t_best) from
+the 10 generator descriptors plus 4 device-computed structure statistics,
+trained on the autotuner's own GPU timings. It only changes which kernel
+runs; its regret is judged against exhaustive tuning, and with `refine` the
+top-k predicted variants are timed and the fastest kept (the fallback).
+Nothing here has a counterpart in the reference.
+"""
+import json
+import math
+import random
+
+import numpy as np
+
+from . import gen, hip
+
+
+def _variants():
+    v = []
+    for bnnz in (64, 128, 256, 512):
+        for lanes in (0, 1):
+            for remap in (0, 1):
+                v.append(("csr_vec", {"block_nnz": bnnz, "lanes_per_row": lanes, "xcd_remap": remap}))
+    for sigma in (64, 256, 1024, 4096, 16384):
+        for unroll in (2, 4, 8):
+            v.append(("sell_c_s", {"sigma": sigma, "unroll": unroll}))
+    for sigma in (32, 128, 512, 2048, 8192):
+        v.append(("sell_c_s", {"sigma": sigma, "unroll": 4}))
+    for unroll in (1, 2, 4, 8):
+        v.append(("sell", {"unroll": unroll}))
+    return v
+
+
+VARIANTS = _variants()
+assert len(VARIANTS) == 40
+
+
+def variant_name(i):
+    k, p = VARIANTS[i]
+    return k + "(" + ",".join("%s=%s" % kv for kv in sorted(p.items())) + ")"
+
+
+# ------------------------------------------------------------------ workloads
+
+def sample_params(master_seed, count, rows=(1e3, 2e5), avg=(4, 64), std_ratio=(0.25, 2.0),
+                  bw=(0.01, 1.0), skew=(1, 100), neigh=(0, 8), crs=(0, 0.9),
+                  distributions=("normal", "gamma"), placements=("random", "diagonal")):
+    """Generator parameter sets drawn from ranges (rows = cols log-uniform);
+    the batched workloads' sampling rule."""
+    rnd = random.Random(master_seed)
+    out = []
+    for _ in range(count):
+        r = int(round(10 ** rnd.uniform(math.log10(rows[0]), math.log10(rows[1]))))
+        a = rnd.uniform(*avg)
+        s = a * rnd.uniform(*std_ratio)
+        d = rnd.choice(distributions)
+        pl = rnd.choice(placements)
+        b = rnd.uniform(*bw)
+        sk = rnd.uniform(*skew)
+        ng = rnd.uniform(*neigh)
+        cr = rnd.uniform(*crs)
+        seed = rnd.randrange(1, 2 ** 31)
+        out.append(gen.GenParams.parse("%d %d %.6g %.6g %s %s %.6g %.6g %.6g %.6g %d" % (
+            r, r, a, s, d, pl, b, sk, ng, cr, seed)))
+    return out
+
+
+# ------------------------------------------------------------------ features
+
+FEATURE_NAMES = ["log_rows", "log_cols", "log_avg", "std_over_avg", "gamma", "diagonal", "bw", "log1p_skew",
+                 "neigh", "crs", "cv_rows", "max_over_avg", "bw_measured", "neigh_per_nnz"]
+
+
+def features(p, st):
+    """14 features: the 10 generator descriptors (log-scaled where they span
+    decades) and 4 device-computed structure statistics."""
+    avg = max(st["avg_nnz_per_row"], 1e-9)
+    return np.array([
+        math.log(p.nr_rows), math.log(p.nr_cols), math.log(max(p.avg_nnz_per_row, 1e-3)),
+        p.std_nnz_per_row / max(p.avg_nnz_per_row, 1e-3),
+        1.0 if p.distribution.startswith("g") else 0.0, 1.0 if p.placement.startswith("d") else 0.0,
+        min(p.bw, 1.0), math.log1p(max(p.skew, 0.0)), p.avg_num_neighbours, p.cross_row_similarity,
+        st["std_nnz_per_row"] / avg, st["max_nnz_per_row"] / avg, st["avg_bw_scaled"],
+        st["avg_num_neighbours"]], dtype=np.float64)
+
+
+class DeviceCSR:
+    """A generated matrix resident on the device, with its x and y."""
+
+    def __init__(self, a, x_seed):
+        self.a = a
+        self.m, self.n, self.nnz = a.m, a.n, a.nnz
+        self.rp = hip.DeviceArray.from_host(a.row_ptr)
+        self.col = hip.DeviceArray.from_host(a.col_ind if a.nnz else np.zeros(1, np.int32))
+        self.val = hip.DeviceArray.from_host(a.values if a.nnz else np.zeros(1))
+        self.x = gen.make_x(a.n, x_seed)
+        self.dx = hip.DeviceArray.from_host(self.x)
+        self.dy = hip.DeviceArray(max(a.m, 1))
+
+    def stats(self):
+        return hip.csr_stats_device(self.rp, self.col, self.m, self.n)
+
+    def convert(self, i):
+        kind, kw = VARIANTS[i]
+        return hip.csr_to_format_device(self.rp, self.col, self.val, self.m, self.n, self.nnz, kind=kind, **kw)
+
+
+def time_variant(dm, i, warmup=2, iters=8):
+    f = dm.convert(i)
+    try:
+        ms = f.time(dm.dx, dm.dy, warmup, iters)
+    finally:
+        f.destroy()
+    return float(np.median(ms))
+
+
+def autotune(dm, variants=None, warmup=2, iters=8):
+    """Exhaustive: median time (ms) of every variant."""
+    idx = range(len(VARIANTS)) if variants is None else variants
+    return np.array([time_variant(dm, i, warmup, iters) for i in idx])
+
+
+# ------------------------------------------------------------------ selector
+
+class Selector:
+    """MLP over standardised features predicting log(t_variant / t_best)."""
+
+    def __init__(self, hidden=(64, 64), seed=0):
+        self.hidden = hidden
+        self.seed = seed
+        self.mu = None
+        self.sd = None
+        self.net = None
+
+    def _norm(self, X):
+        return (np.asarray(X, dtype=np.float64) - self.mu) / self.sd
+
+    @staticmethod
+    def targets(times):
+        t = np.asarray(times, dtype=np.float64)
+        return np.log(t / t.min(axis=1, keepdims=True))
+
+    def fit(self, X, times, epochs=300, batch=128, lr=2e-3, shuffle_seed=0):
+        X = np.asarray(X, dtype=np.float64)
+        self.mu = X.mean(axis=0)
+        self.sd = X.std(axis=0) + 1e-9
+        T = self.targets(times)
+        self.net = hip.MLP(X.shape[1], self.hidden[0], self.hidden[1], T.shape[1], seed=self.seed)
+        rng = np.random.default_rng(shuffle_seed)
+        Xn = self._norm(X).astype(np.float32)
+        losses = []
+        for _ in range(epochs):
+            order = rng.permutation(len(X))
+            losses.append(float(self.net.train(Xn[order], T[order].astype(np.float32), batch=batch,
+                                               epochs=1, lr=lr)[0]))
+        return np.array(losses)
+
+    def predict(self, X):
+        return self.net.predict(self._norm(np.atleast_2d(X)).astype(np.float32))
+
+    def select(self, X, k=1):
+        """Indices of the k best predicted variants per row."""
+        return np.argsort(self.predict(X), axis=1)[:, :k]
+
+    def save(self, path):
+        json.dump({"hidden": list(self.hidden), "mu": self.mu.tolist(), "sd": self.sd.tolist(),
+                   "dims": list(self.net.dims), "params": self.net.get_params().tolist(),
+                   "variants": [variant_name(i) for i in range(len(VARIANTS))]}, open(path, "w"))
+
+    @classmethod
+    def load(cls, path):
+        d = json.load(open(path))
+        s = cls(tuple(d["hidden"]))
+        s.mu = np.array(d["mu"])
+        s.sd = np.array(d["sd"])
+        s.net = hip.MLP(*d["dims"])
+        s.net.set_params(np.array(d["params"], dtype=np.float32))
+        return s
+
+
+def regret(times_row, choice):
+    return float(times_row[choice] / np.min(times_row) - 1.0)
