@@ -73,8 +73,117 @@ def cpu_baseline(a, x, y_gpu, budget_s):
             "gflops": 2 * a.nnz / med / 1e9}, err
 
 
+def parse_format(tok):
+    """kind[:sigma] or kind:key=value;key=value"""
+    k, _, rest = tok.partition(":")
+    if not rest:
+        return k, {}
+    if "=" not in rest:
+        return k, {"sigma": int(rest)}
+    return k, {kv.split("=")[0]: int(kv.split("=")[1]) for kv in rest.split(";")}
+
+
+def probe(hip, np):
+    """Stream-read and copy bandwidth of 2 GiB buffers: the measured ceiling
+    and the known traffic for calibrating the profiler's byte counters."""
+    n = 1 << 28  # doubles = 2 GiB
+    src = hip.DeviceArray(n)
+    dst = hip.DeviceArray(n)
+    src.fill_bytes(0)
+    out = {}
+    for mode, name, nbytes in ((0, "stream_read", 8 * n), (1, "stream_copy", 16 * n)):
+        ms = hip.bandwidth_probe(mode, src, dst if mode == 1 else None, 20)
+        out[name] = {"bytes": nbytes, "median_ms": float(np.median(ms)),
+                     "gbps": nbytes / (float(np.median(ms)) * 1e-3) / 1e9}
+        log("%s: %.3f ms, %.1f GB/s" % (name, out[name]["median_ms"], out[name]["gbps"]))
+    print(json.dumps({"probe": out}), flush=True)
+
+
+def run_batched(args, gen, hip, np):
+    """Batched workload: `count` generated matrices of uneven sizes (rows =
+    cols log-uniform in [1e3, 2e5], avg 4..64, ...; master seed 21), each with
+    its own x, computed by one persistent work-queue launch replayed from a
+    hipGraph, with every matrix's format chosen by the learned selector;
+    compared on the same GPU with plain per-matrix binned-CSR launches.
+    Prints one JSON line (matrices/s)."""
+    from spmvdare import tuning
+    count = args.count
+    model = os.path.join(ROOT, "spmv-dare_amd", "spmvdare", "models", "selector.json")
+    sel = tuning.Selector.load(model) if os.path.exists(model) and not args.no_selector else None
+    fixed = tuning.VARIANTS.index(("sell_c_s", {"sigma": 256, "lanes_per_row": 4, "unroll": 4}))
+    csr = tuning.VARIANTS.index(("csr_vec", {"block_nnz": 512, "lanes_per_row": 0}))
+    params = tuning.sample_params(21, count * 2)
+    t0 = time.perf_counter()
+    members, total_nnz, total_bytes, chosen = [], 0, 0, []
+    for p in params:
+        if len(members) == count:
+            break
+        try:
+            a = gen.generate(p, stats=False)
+        except gen.GenError:
+            continue
+        dm = tuning.DeviceCSR(a, p.seed + 1)
+        if sel is not None:
+            v = int(sel.select(tuning.features(p, dm.stats()))[0, 0])
+        else:
+            v = fixed
+        chosen.append(v)
+        f_sel = dm.convert(v)
+        f_csr = dm.convert(csr)
+        dy_csr = hip.DeviceArray(max(a.m, 1))
+        members.append((dm, f_sel, f_csr, dy_csr))
+        total_nnz += a.nnz
+        total_bytes += alg_bytes(a.m, a.n, a.nnz)
+        dm.a = None  # host arrays no longer needed
+        if len(members) % 256 == 0:
+            log("batched: %d/%d matrices prepared, %.0f s" % (len(members), count, time.perf_counter() - t0))
+    log("batched: %d matrices, %d nnz, prepared in %.0f s; selector=%s" % (
+        len(members), total_nnz, time.perf_counter() - t0, "mlp" if sel is not None else "fixed"))
+    batch = hip.Batch(args.tile)
+    base = hip.Batch(args.tile)
+    for dm, f_sel, f_csr, dy_csr in members:
+        batch.add(f_sel, dm.dx, dm.dy)
+        base.add(f_csr, dm.dx, dy_csr)
+    batch.finalize(use_graph=True)
+    base.finalize(use_graph=False)
+    hip.synchronize()
+    ms_batch = batch.time(args.warmup, args.steps)
+    ms_plain = base.time(max(1, args.warmup // 2), max(3, args.steps // 4), per_matrix=True)
+    # consistency: every member's batched y against its own per-matrix CSR y
+    worst = 0.0
+    for dm, f_sel, f_csr, dy_csr in members:
+        yb, yc = dm.dy.download(), dy_csr.download()
+        nrm = np.linalg.norm(yc)
+        worst = max(worst, float(np.linalg.norm(yb - yc) / nrm) if nrm > 0 else float(np.linalg.norm(yb)))
+    t_b, t_p = float(np.median(ms_batch)), float(np.median(ms_plain))
+    hist = np.bincount(np.array(chosen), minlength=len(tuning.VARIANTS))
+    out = {
+        "metric": "batched FP64 SpMV matrices/s (selector-chosen formats, one persistent hipGraph launch)",
+        "value": round(len(members) / (t_b * 1e-3), 1), "unit": "matrices/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_b, 4), "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic (reference generator restated bit-exactly)",
+        "config": {"workload": "%d generated matrices, master seed 21" % len(members), "nnz": total_nnz,
+                   "alg_bytes": total_bytes, "tile_units": args.tile,
+                   "selector": "mlp" if sel is not None else tuning.variant_name(fixed)},
+        "batched_gbps": round(total_bytes / (t_b * 1e-3) / 1e9, 1),
+        "plain_per_matrix_csr": {"ms": round(t_p, 4), "matrices_per_s": round(len(members) / (t_p * 1e-3), 1),
+                                 "gbps": round(total_bytes / (t_p * 1e-3) / 1e9, 1)},
+        "speedup_vs_plain_csr": round(t_p / t_b, 3),
+        "max_normwise_diff_vs_csr": worst,
+        "chosen_variants": {tuning.variant_name(i): int(c) for i, c in enumerate(hist) if c},
+    }
+    log("batched %.3f ms (%.0f matrices/s, %.0f GB/s) vs plain per-matrix CSR %.3f ms: x%.2f" % (
+        t_b, out["value"], out["batched_gbps"], t_p, t_p / t_b))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--probe", action="store_true", help="measure stream read / copy bandwidth only")
+    ap.add_argument("--batched", action="store_true", help="batched workload instead of one matrix")
+    ap.add_argument("--count", type=int, default=2048, help="matrices of the batched workload")
+    ap.add_argument("--tile", type=int, default=16, help="units per work item of the batched launch")
+    ap.add_argument("--no-selector", action="store_true")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=128)
     ap.add_argument("--warmup", type=int, default=10)
@@ -96,6 +205,12 @@ def main():
 
     hip.init(0)
     log("device:", hip.device_name())
+    if args.probe:
+        probe(hip, np)
+        return
+    if args.batched:
+        run_batched(args, gen, hip, np)
+        return
     line, desc = WORKLOADS[args.workload]
     p = gen.GenParams.parse(line)
     t0 = time.perf_counter()
@@ -105,11 +220,11 @@ def main():
     B = alg_bytes(a.m, a.n, a.nnz)
 
     cands = CANDIDATES
-    if args.formats != "all":
-        cands = []
-        for tok in args.formats.split(","):
-            k, _, s = tok.partition(":")
-            cands.append((k, {"sigma": int(s)} if s else {}))
+    if args.formats == "sweep":
+        from spmvdare import tuning
+        cands = list(tuning.VARIANTS) + [("csr_naive", {})]
+    elif args.formats != "all":
+        cands = [parse_format(tok) for tok in args.formats.split(",")]
 
     dx = hip.DeviceArray.from_host(x)
     dy = hip.DeviceArray(a.m)

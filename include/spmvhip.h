@@ -67,12 +67,13 @@ typedef struct spmvhip_params {
 	int kind;             /* enum spmvhip_kind */
 	int C;                /* SELL slice height; must be 64 (one wavefront) -- default 64 */
 	long sigma;           /* SELL-C-sigma sort window, 1..16384 -- default 4096 */
-	int lanes_per_row;    /* CSR_VEC: 0 = auto; else lanes of the LDS reduction per row (1..64) */
-	int block_nnz;        /* CSR_VEC: nnz per row block staged in LDS (64..1024, default 512) */
+	int lanes_per_row;    /* CSR_VEC: 0 = auto, else lanes of the LDS reduction per row (1..64);
+	                         SELL*: lanes per row T (power of two, 0/1 = the reference layout) */
+	int block_nnz;        /* CSR_VEC: nnz per row block staged in LDS (64..1024, default 256) */
 	int split_nnz;        /* CSR_VEC: rows longer than this are split across waves (default 8192) */
 	int xcd_remap;        /* 1 = give each XCD a contiguous range of work (default 0) */
 	int x_window;         /* SELL*: 1 = stage the slice's x window in LDS when it fits */
-	int unroll;           /* SELL*: inner unroll (1, 2, 4, 8; default 4) */
+	int unroll;           /* SELL*: inner unroll (1, 2, 4, 8; default 8) */
 	int div_threads;      /* DIV_RF: encoder partitions T (default 1) */
 	int div_lanes;        /* DIV_RF: lanes L of the value streams (default 64) */
 	long div_packet_vals; /* DIV_RF: P, max values per packet (default 16384) */
@@ -181,6 +182,12 @@ int spmvhip_batch_synchronize(spmvhip_batch *b);
 int spmvhip_batch_time(spmvhip_batch *b, int per_matrix, int warmup, int iters, float *ms);
 int spmvhip_batch_info(const spmvhip_batch *b, long *nmats, long *nitems, int *grid, long *total_nnz);
 void spmvhip_batch_destroy(spmvhip_batch *b);
+
+/* Bandwidth reference (no counterpart in the reference): mode 0 streams n
+ * doubles of src once (16-byte loads), mode 1 copies src to dst; ms[i] is
+ * iteration i's time. The measured ceiling for the SpMV kernels and the
+ * known-size traffic that calibrates the profiler's byte counters. */
+int spmvhip_bandwidth_probe(int mode, const double *src, double *dst, long n, int iters, float *ms);
 
 /* ---- format selector support (no counterpart in the reference) ---- */
 

@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(kBlock) batch_kernel(const BatchMat *__restric
 						M.lanes, prod[wv]);
 		} else {
 			for (int u = wi.u0 + wv; u < wi.u1; u += kWavesPerBlock)
-				sell_slice<4>(M.sptr, M.sw, M.scol, M.sval, M.order, M.x, M.y, M.m, u);
+				sell_slice<4>(M.sptr, M.sw, M.scol, M.sval, M.order, M.x, M.y, M.m, u, M.lanes);
 		}
 	}
 }

@@ -133,7 +133,7 @@ int csr_vec_convert(spmvhip_format *f)
 {
 	hipStream_t s = f->stream;
 	const long m = f->m;
-	f->block_nnz = f->p.block_nnz > 0 ? f->p.block_nnz : 512;
+	f->block_nnz = f->p.block_nnz > 0 ? f->p.block_nnz : 256;
 	f->split_nnz = f->p.split_nnz > 0 ? f->p.split_nnz : 8192;
 	if (f->block_nnz < 64 || f->block_nnz > 1024)
 		return set_error(SPMVHIP_EINVAL, "csr_vec: block_nnz %d outside [64, 1024]", f->block_nnz);

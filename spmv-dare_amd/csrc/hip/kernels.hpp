@@ -49,8 +49,26 @@ __device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const 
 			y[r0] = sum;
 		return;
 	}
-	// stream block: products to LDS, coalesced over the block's nnz
-	for (int e = lane; e < nz; e += kWave)
+	// stream block: products to LDS, coalesced over the block's nnz, four
+	// loads per lane in flight
+	int e = lane;
+	for (; e + 3 * kWave < nz; e += 4 * kWave) {
+		double v[4];
+		int32_t c[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			v[u] = __builtin_nontemporal_load(val + j0 + e + u * kWave);
+			c[u] = __builtin_nontemporal_load(col + j0 + e + u * kWave);
+		}
+		double xv[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++)
+			xv[u] = x[c[u]];
+#pragma unroll
+		for (int u = 0; u < 4; u++)
+			p[e + u * kWave] = v[u] * xv[u];
+	}
+	for (; e < nz; e += kWave)
 		p[e] = val[j0 + e] * x[col[j0 + e]];
 	wave_lds_sync();
 	// lanes per row: largest power of two <= 64 / nrows (or the configured value)
@@ -74,41 +92,61 @@ __device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const 
 	wave_lds_sync(); // the buffer is reused by the wave's next block
 }
 
-// One SELL slice of 64 rows, lane = row: one fused multiply-add per stored
-// entry in entry order; UNROLL entries are loaded ahead of their FMAs.
+// One SELL slice of 64 rows with T lanes per row (T a power of two; the
+// slice is processed in T rounds of 64 / T rows). Every wave step reads 64
+// consecutive stored entries: with T = 1 these are entry k of all 64 rows
+// (the reference's column-major slice), with T > 1 entries jT..jT+T-1 of
+// 64 / T rows, so x is gathered in runs of T along each row. Each lane keeps
+// one fused multiply-add chain in entry order; UNROLL entries are loaded
+// ahead of their FMAs; the T partial sums of a row are added at the end.
+// With ystage != nullptr the results go to ystage[target - row0] (LDS)
+// instead of y[target]: the caller writes the staged rows out coalesced.
 template <int UNROLL>
 __device__ __forceinline__ void sell_slice(const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
 		const int32_t *__restrict__ scol, const double *__restrict__ sval, const int32_t *__restrict__ order,
-		const double *__restrict__ x, double *__restrict__ y, long m, long s)
+		const double *__restrict__ x, double *__restrict__ y, long m, long s, int T, double *ystage = nullptr,
+		long row0 = 0)
 {
 	constexpr int C = kWave;
 	const int lane = lane_id();
 	const int w = width[s];
-	const double *v = sval + sptr[s] + lane;
-	const int32_t *c = scol + sptr[s] + lane;
-	double acc = 0;
-	int k = 0;
-	for (; k + UNROLL <= w; k += UNROLL) {
-		double vv[UNROLL];
-		int32_t cc[UNROLL];
+	const int wp = (w + T - 1) / T * T;
+	const int steps = wp / T, R = C / T;
+	for (int q = 0; q < T; q++) {
+		const long off = sptr[s] + (long) q * R * wp + lane;
+		const double *v = sval + off;
+		const int32_t *c = scol + off;
+		double acc = 0;
+		int k = 0;
+		for (; k + UNROLL <= steps; k += UNROLL) {
+			double vv[UNROLL];
+			int32_t cc[UNROLL];
 #pragma unroll
-		for (int u = 0; u < UNROLL; u++) {
-			vv[u] = __builtin_nontemporal_load(v + (long) (k + u) * C);
-			cc[u] = __builtin_nontemporal_load(c + (long) (k + u) * C);
+			for (int u = 0; u < UNROLL; u++) {
+				vv[u] = __builtin_nontemporal_load(v + (long) (k + u) * C);
+				cc[u] = __builtin_nontemporal_load(c + (long) (k + u) * C);
+			}
+			double xx[UNROLL];
+#pragma unroll
+			for (int u = 0; u < UNROLL; u++)
+				xx[u] = x[cc[u]];
+#pragma unroll
+			for (int u = 0; u < UNROLL; u++)
+				acc = fma(vv[u], xx[u], acc);
 		}
-		double xx[UNROLL];
-#pragma unroll
-		for (int u = 0; u < UNROLL; u++)
-			xx[u] = x[cc[u]];
-#pragma unroll
-		for (int u = 0; u < UNROLL; u++)
-			acc = fma(vv[u], xx[u], acc);
+		for (; k < steps; k++)
+			acc = fma(v[(long) k * C], x[c[(long) k * C]], acc);
+		if (T > 1)
+			acc = group_sum(acc, T);
+		const long r = s * C + (long) q * R + lane / T;
+		if ((lane & (T - 1)) == 0 && r < m) {
+			const long tgt = order ? order[r] : r;
+			if (ystage)
+				ystage[tgt - row0] = acc;
+			else
+				y[tgt] = acc;
+		}
 	}
-	for (; k < w; k++)
-		acc = fma(v[(long) k * C], x[c[(long) k * C]], acc);
-	const long r = s * C + lane;
-	if (r < m)
-		y[order ? order[r] : r] = acc;
 }
 
 } // namespace spmvhip

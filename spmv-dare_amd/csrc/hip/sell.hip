@@ -125,9 +125,10 @@ __global__ void __launch_bounds__(kSortThreads) sigma_window_sort(const int32_t 
 		order[w0 + i] = ids[i];
 }
 
-// Slice widths (longest row) and slice sizes width * 64.
+// Slice widths (longest row) and slice sizes: width rounded up to a multiple
+// of the lanes per row T, times 64.
 __global__ void slice_widths(const int32_t *__restrict__ len, const int32_t *__restrict__ order, long m, long nslices,
-		int32_t *__restrict__ width, int64_t *__restrict__ size)
+		int T, int32_t *__restrict__ width, int64_t *__restrict__ size)
 {
 	long s = (long) blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= nslices)
@@ -141,7 +142,7 @@ __global__ void slice_widths(const int32_t *__restrict__ len, const int32_t *__r
 		w = k > w ? k : w;
 	}
 	width[s] = w;
-	size[s] = (int64_t) w * kSliceC;
+	size[s] = (int64_t) ((w + T - 1) / T * T) * kSliceC;
 }
 
 // last_ne[r] = largest r' <= r with a non-empty row r' (or -1): input of the
@@ -155,16 +156,19 @@ __global__ void nonempty_keys(const int32_t *__restrict__ len, long m, int64_t *
 
 __global__ void __launch_bounds__(kBlock) sell_fill(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
 		const double *__restrict__ val, const int32_t *__restrict__ order, const int64_t *__restrict__ last_ne,
-		const int32_t *__restrict__ width, const int64_t *__restrict__ sptr, long m, long nslices,
+		const int32_t *__restrict__ width, const int64_t *__restrict__ sptr, long m, long nslices, int T,
 		int32_t *__restrict__ scol, double *__restrict__ sval)
 {
 	const long s = (long) blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
 	if (s >= nslices)
 		return;
-	const int lane = lane_id();
+	const int lane = lane_id();          // the slice row this lane fills
 	const long r = s * kSliceC + lane;
 	const int w = width[s];
-	const int64_t base = sptr[s];
+	const int wp = (w + T - 1) / T * T; // width padded to the lanes per row
+	const int R = kSliceC / T;          // rows per round
+	// entry k of slice row `lane`: round lane / R, lane group lane % R, step k / T
+	const int64_t base = sptr[s] + (int64_t) (lane / R) * R * wp + (lane % R) * T;
 	int j0 = 0, len = 0, pad_col = 0;
 	if (r < m) {
 		long src = order ? order[r] : r;
@@ -176,9 +180,11 @@ __global__ void __launch_bounds__(kBlock) sell_fill(const int32_t *__restrict__ 
 		long rr = r < m ? r : m - 1;
 		int64_t ne = m > 0 ? last_ne[rr] : -1;
 		pad_col = ne >= 0 ? col[rp[ne + 1] - 1] : 0;
+	} else if (T > 1 && len > 0) {
+		pad_col = col[j0 + len - 1]; // keeps the padding's x reads on the row's own lines
 	}
-	for (int k = 0; k < w; k++) {
-		int64_t idx = base + (int64_t) k * kSliceC + lane;
+	for (int k = 0; k < wp; k++) {
+		int64_t idx = base + (int64_t) (k / T) * kSliceC + (k % T);
 		if (k < len) {
 			scol[idx] = col[j0 + k];
 			sval[idx] = val[j0 + k];
@@ -199,6 +205,10 @@ int sell_convert(spmvhip_format *f, bool sorted)
 	if (sorted && sigma > kMaxSigma)
 		return set_error(SPMVHIP_EINVAL, "sell_c_s: sigma = %ld > %d (window must fit one workgroup's LDS)",
 				sigma, kMaxSigma);
+	const int T = f->p.lanes_per_row > 0 ? f->p.lanes_per_row : 1;
+	if (T > kSliceC || (T & (T - 1)))
+		return set_error(SPMVHIP_EINVAL, "sell: lanes_per_row %d is not a power of two in [1, 64]", T);
+	f->lanes = T;
 	f->p.C = kSliceC;
 	f->p.sigma = sorted ? sigma : 1;
 	f->nslices = (m + kSliceC - 1) / kSliceC;
@@ -223,7 +233,7 @@ int sell_convert(spmvhip_format *f, bool sorted)
 	}
 	SPMVHIP_TRY(f->slice_w.alloc(f->nslices * sizeof(int32_t)));
 	SPMVHIP_TRY(f->slice_ptr.alloc((f->nslices + 1) * sizeof(int64_t)));
-	slice_widths<<<grid_for(f->nslices, kBlock), kBlock, 0, s>>>(len.as<int32_t>(), order, m, f->nslices,
+	slice_widths<<<grid_for(f->nslices, kBlock), kBlock, 0, s>>>(len.as<int32_t>(), order, m, f->nslices, T,
 			f->slice_w.as<int32_t>(), sizes.as<int64_t>());
 	SPMVHIP_CHECK(hipGetLastError());
 	SPMVHIP_TRY((device_scan<int64_t, OpSum>(sizes.as<int64_t>(), f->slice_ptr.as<int64_t>(), f->nslices, 0, 1, s)));
@@ -246,7 +256,7 @@ int sell_convert(spmvhip_format *f, bool sorted)
 	SPMVHIP_TRY(f->s_vals.alloc(total * sizeof(double)));
 	sell_fill<<<grid_for(f->nslices, kWavesPerBlock), kBlock, 0, s>>>(f->row_ptr.as<int32_t>(), f->col.as<int32_t>(),
 			f->val.as<double>(), order, sorted ? nullptr : last_ne.as<int64_t>(), f->slice_w.as<int32_t>(),
-			f->slice_ptr.as<int64_t>(), m, f->nslices, f->s_cols.as<int32_t>(), f->s_vals.as<double>());
+			f->slice_ptr.as<int64_t>(), m, f->nslices, T, f->s_cols.as<int32_t>(), f->s_vals.as<double>());
 	SPMVHIP_CHECK(hipGetLastError());
 	SPMVHIP_CHECK(hipStreamSynchronize(s));
 	// the CSR copy is no longer needed once the slices exist
@@ -262,17 +272,30 @@ int sell_convert(spmvhip_format *f, bool sorted)
 template <int UNROLL>
 __global__ void __launch_bounds__(kBlock) sell_kernel(const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
 		const int32_t *__restrict__ scol, const double *__restrict__ sval, const int32_t *__restrict__ order,
-		const double *__restrict__ x, double *__restrict__ y, long m, long nslices, int remap)
+		const double *__restrict__ x, double *__restrict__ y, long m, long nslices, int T, int remap, int stage)
 {
+	__shared__ double ybuf[kWavesPerBlock * kSliceC];
 	long blk = blockIdx.x;
 	if (remap) {
 		long per = gridDim.x / 8;
 		blk = (long) (blockIdx.x % 8) * per + blockIdx.x / 8;
 	}
 	const long s = blk * kWavesPerBlock + threadIdx.x / kWave;
-	if (s >= nslices)
+	if (!stage) {
+		if (s < nslices)
+			sell_slice<UNROLL>(sptr, width, scol, sval, order, x, y, m, s, T);
 		return;
-	sell_slice<UNROLL>(sptr, width, scol, sval, order, x, y, m, s);
+	}
+	// The workgroup's 4 slices are rows [row0, row0 + 256) and every target
+	// row of the permutation stays inside them (sigma divides 256): stage the
+	// results in LDS and store them as one contiguous run.
+	const long row0 = blk * kWavesPerBlock * kSliceC;
+	if (s < nslices)
+		sell_slice<UNROLL>(sptr, width, scol, sval, order, x, y, m, s, T, ybuf, row0);
+	__syncthreads();
+	const long r = row0 + threadIdx.x;
+	if (r < m && row0 < m)
+		y[r] = ybuf[threadIdx.x];
 }
 
 int sell_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
@@ -284,10 +307,15 @@ int sell_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 	if (remap)
 		grid = (grid + 7) / 8 * 8;
 	const int32_t *order = f->kind == SPMVHIP_SELL_C_S ? f->row_order.as<int32_t>() : nullptr;
-	int u = f->p.unroll > 0 ? f->p.unroll : 4;
+	int u = f->p.unroll > 0 ? f->p.unroll : 8;
+	// y through LDS when the permutation cannot leave a workgroup's 256 rows
+	// (sigma divides 256) or several lanes per row would store it piecewise
+	const long wg_rows = (long) kWavesPerBlock * kSliceC;
+	const int stage = (order && wg_rows % f->p.sigma == 0) || (!order && f->lanes > 1) ? 1 : 0;
 #define LAUNCH_SELL(U)                                                                                       \
 	sell_kernel<U><<<grid, kBlock, 0, s>>>(f->slice_ptr.as<int64_t>(), f->slice_w.as<int32_t>(),            \
-			f->s_cols.as<int32_t>(), f->s_vals.as<double>(), order, x, y, f->m, f->nslices, remap)
+			f->s_cols.as<int32_t>(), f->s_vals.as<double>(), order, x, y, f->m, f->nslices, f->lanes, remap, \
+			stage)
 	if (u >= 8)
 		LAUNCH_SELL(8);
 	else if (u >= 4)

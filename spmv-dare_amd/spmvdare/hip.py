@@ -78,6 +78,7 @@ _sig = {
                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
     "spmvhip_batch_destroy": ([_vp], None),
     "spmvhip_csr_stats_device": ([_vp, _vp, ctypes.c_long, ctypes.c_long, _vp], ctypes.c_int),
+    "spmvhip_bandwidth_probe": ([ctypes.c_int, _vp, _vp, ctypes.c_long, ctypes.c_int, _vp], ctypes.c_int),
     "spmvhip_mlp_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint,
                             ctypes.POINTER(_vp)], ctypes.c_int),
     "spmvhip_mlp_num_params": ([_vp], ctypes.c_long),
@@ -315,6 +316,14 @@ class Batch:
             self.destroy()
         except Exception:
             pass
+
+
+def bandwidth_probe(mode, src, dst, iters):
+    """Times of `iters` stream reads (mode 0) or copies (mode 1) of src."""
+    ms = (ctypes.c_float * iters)()
+    _check(_lib.spmvhip_bandwidth_probe(mode, src.ptr, dst.ptr if dst is not None else None, src.size,
+                                        iters, ms), "bandwidth_probe")
+    return np.array(ms[:], dtype=np.float64)
 
 
 def csr_stats_device(d_row_ptr, d_col_ind, m, n):

@@ -23,17 +23,27 @@ from . import gen, hip
 
 def _variants():
     v = []
+    # binned CSR: rows per wave through the row-block size, lanes per row in the reduction
     for bnnz in (64, 128, 256, 512):
         for lanes in (0, 1):
-            for remap in (0, 1):
-                v.append(("csr_vec", {"block_nnz": bnnz, "lanes_per_row": lanes, "xcd_remap": remap}))
-    for sigma in (64, 256, 1024, 4096, 16384):
-        for unroll in (2, 4, 8):
-            v.append(("sell_c_s", {"sigma": sigma, "unroll": unroll}))
-    for sigma in (32, 128, 512, 2048, 8192):
-        v.append(("sell_c_s", {"sigma": sigma, "unroll": 4}))
-    for unroll in (1, 2, 4, 8):
-        v.append(("sell", {"unroll": unroll}))
+            v.append(("csr_vec", {"block_nnz": bnnz, "lanes_per_row": lanes}))
+    for bnnz in (128, 256):
+        v.append(("csr_vec", {"block_nnz": bnnz, "xcd_remap": 1}))
+    # SELL-C-sigma, C = 64: sort window sigma x lanes per row T
+    for sigma in (64, 256, 1024, 4096):
+        for T in (1, 2, 4, 8):
+            v.append(("sell_c_s", {"sigma": sigma, "lanes_per_row": T, "unroll": 4}))
+    for T in (1, 4):
+        v.append(("sell_c_s", {"sigma": 16384, "lanes_per_row": T, "unroll": 4}))
+    for unroll in (2, 8):
+        v.append(("sell_c_s", {"sigma": 256, "lanes_per_row": 1, "unroll": unroll}))
+    for sigma in (32, 128, 512, 2048):
+        v.append(("sell_c_s", {"sigma": sigma, "lanes_per_row": 4, "unroll": 4}))
+    # unsorted SELL
+    for T in (1, 2, 4, 8):
+        v.append(("sell", {"lanes_per_row": T, "unroll": 4}))
+    for unroll in (2, 8):
+        v.append(("sell", {"lanes_per_row": 1, "unroll": unroll}))
     return v
 
 

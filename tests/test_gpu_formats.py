@@ -178,6 +178,32 @@ def test_sell_c_sigma_unaligned_window(name, sigma):
     np.testing.assert_array_equal(y, pyoracle.sellcs_spmv(ref, 64, x))
 
 
+@pytest.mark.parametrize("name", ["test_txt", "cfg1", "ragged_4097", "huge_rows", "block_edges", "one_row_empty",
+                                  "run_sh_2", "dense_rows"])
+@pytest.mark.parametrize("kind,T", [("sell_c_s", 2), ("sell_c_s", 8), ("sell_c_s", 64), ("sell", 4)])
+def test_sell_lanes_per_row(name, kind, T):
+    """T lanes per row: same permutation and slices, entries regrouped so that
+    a wave step reads T consecutive entries of 64 / T rows; T = 1 is the
+    reference layout (bit-exact tests above), T > 1 changes the summation
+    order, so y is checked normwise."""
+    rp, ci, va, n = matrix(name)
+    m = rp.size - 1
+    x = gen.make_x(n, 94)
+    f = hip.csr_to_format(rp, ci, va, m, n, ci.size, kind=kind, sigma=256, lanes_per_row=T)
+    lens = np.diff(rp)
+    if kind == "sell_c_s" and m > 0:
+        np.testing.assert_array_equal(f.export(0, np.int32), pyoracle.sellcs_convert(rp, ci, va, 64, 256)["order"])
+    widths = f.export(1, np.int32)
+    sptr = f.export(2, np.int64)
+    np.testing.assert_array_equal(np.diff(sptr), ((widths + T - 1) // T * T) * 64)
+    vals = f.export(4, np.float64)
+    assert np.count_nonzero(vals) == np.count_nonzero(va)
+    y = run(f, x, m)
+    assert pyoracle.normwise_rel_err(y, pyoracle.csr_naive(rp, ci, va, x)) <= TOL
+    assert (y[lens == 0] == 0).all()
+    check_dd(y, rp, ci, va, x, "%s T%d/%s" % (kind, T, name))
+
+
 @pytest.mark.parametrize("name", ALL)
 @pytest.mark.parametrize("unroll", [1, 4, 8])
 def test_sell_bit_exact(name, unroll):
