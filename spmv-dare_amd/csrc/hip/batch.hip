@@ -153,6 +153,7 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 		M.scol = f->s_cols.as<int32_t>();
 		M.sval = f->s_vals.as<double>();
 		M.order = f->kind == SPMVHIP_SELL_C_S ? f->row_order.as<int32_t>() : nullptr;
+		M.lanes = f->lanes; // lanes per row T (>= 1) of the slice layout
 		units = f->nslices;
 	} else {
 		return set_error(SPMVHIP_EINVAL, "batch: format %s is not batchable (csr_vec, sell, sell_c_s are)",

@@ -196,16 +196,6 @@ int csr_vec_convert(spmvhip_format *f)
 
 // ------------------------------------------------------------------ binned kernels
 
-// Map a workgroup index so that each of the 8 XCDs (which receive workgroups
-// round-robin) works on one contiguous eighth of the work list.
-__device__ __forceinline__ long xcd_logical_block(int remap)
-{
-	if (!remap)
-		return blockIdx.x;
-	long per = gridDim.x / 8; // grid is padded to a multiple of 8 when remapping
-	return (long) (blockIdx.x % 8) * per + blockIdx.x / 8;
-}
-
 template <int BNNZ_MAX>
 __global__ void __launch_bounds__(kBlock) csr_binned_kernel(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
 		const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
@@ -252,10 +242,8 @@ int csr_vec_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 				f->val.as<double>(), x, f->seg_desc.as<int32_t>(), f->nsegs, f->partials.as<double>());
 		SPMVHIP_CHECK(hipGetLastError());
 	}
-	unsigned grid = grid_for(f->nblocks, kWavesPerBlock);
-	int remap = f->p.xcd_remap ? 1 : 0;
-	if (remap)
-		grid = (grid + 7) / 8 * 8;
+	const int remap = xcd_chunk(f->p.xcd_remap);
+	const unsigned grid = xcd_padded_grid(grid_for(f->nblocks, kWavesPerBlock), remap);
 #define LAUNCH_BINNED(B)                                                                                    \
 	csr_binned_kernel<B><<<grid, kBlock, 0, s>>>(f->row_ptr.as<int32_t>(), f->col.as<int32_t>(),        \
 			f->val.as<double>(), x, y, f->blocks.as<int32_t>(), f->nblocks, f->split_nnz, f->lanes, \

@@ -6,6 +6,36 @@
 
 namespace spmvhip {
 
+// Workgroups are handed to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8). With a chunk size K > 0 the work list is cut into chunks of K
+// workgroups and XCD x takes chunks x, x + 8, x + 16, ...: each XCD's L2
+// then holds the x lines of one contiguous range of rows at a time while
+// the chunks keep the XCDs' loads even. The grid must be padded to a
+// multiple of 8 K; K = 0 is the identity.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ long xcd_logical_block(int K)
+{
+	if (K <= 0)
+		return blockIdx.x;
+	const long i = blockIdx.x / kXcds, x = blockIdx.x % kXcds;
+	return (i / K) * (long) kXcds * K + x * (long) K + (i % K);
+}
+
+inline unsigned xcd_padded_grid(unsigned grid, int K)
+{
+	if (K <= 0)
+		return grid;
+	const unsigned q = (unsigned) kXcds * (unsigned) K;
+	return (grid + q - 1) / q * q;
+}
+
+// the remap parameter: 0 off, 1 the default chunk, n > 1 chunks of n workgroups
+inline int xcd_chunk(int remap)
+{
+	return remap <= 0 ? 0 : (remap == 1 ? 64 : remap);
+}
+
 __device__ __forceinline__ void wave_lds_sync()
 {
 	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

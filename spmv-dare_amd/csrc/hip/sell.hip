@@ -275,11 +275,7 @@ __global__ void __launch_bounds__(kBlock) sell_kernel(const int64_t *__restrict_
 		const double *__restrict__ x, double *__restrict__ y, long m, long nslices, int T, int remap, int stage)
 {
 	__shared__ double ybuf[kWavesPerBlock * kSliceC];
-	long blk = blockIdx.x;
-	if (remap) {
-		long per = gridDim.x / 8;
-		blk = (long) (blockIdx.x % 8) * per + blockIdx.x / 8;
-	}
+	const long blk = xcd_logical_block(remap);
 	const long s = blk * kWavesPerBlock + threadIdx.x / kWave;
 	if (!stage) {
 		if (s < nslices)
@@ -302,10 +298,8 @@ int sell_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 {
 	if (f->m == 0)
 		return SPMVHIP_OK;
-	unsigned grid = grid_for(f->nslices, kWavesPerBlock);
-	int remap = f->p.xcd_remap ? 1 : 0;
-	if (remap)
-		grid = (grid + 7) / 8 * 8;
+	const int remap = xcd_chunk(f->p.xcd_remap);
+	const unsigned grid = xcd_padded_grid(grid_for(f->nslices, kWavesPerBlock), remap);
 	const int32_t *order = f->kind == SPMVHIP_SELL_C_S ? f->row_order.as<int32_t>() : nullptr;
 	int u = f->p.unroll > 0 ? f->p.unroll : 8;
 	// y through LDS when the permutation cannot leave a workgroup's 256 rows

@@ -18,13 +18,17 @@ import numpy as np
 from . import gen, hip, tuning
 
 
-def build(count, seed, warmup=2, iters=8, log=None, ranges=None):
+def build(count, seed, warmup=2, iters=8, log=None, ranges=None, budget_s=None):
     params = tuning.sample_params(seed, count * 2, **(ranges or {}))
     feats, times, lines, nnzs = [], [], [], []
     t0 = time.time()
     k = 0
     for p in params:
         if len(lines) == count:
+            break
+        if budget_s is not None and time.time() - t0 > budget_s:
+            if log:
+                log("corpus: time budget reached after %d matrices" % k)
             break
         try:
             a = gen.generate(p, stats=False)
@@ -49,13 +53,14 @@ def main():
     ap.add_argument("--seed", type=int, default=33)
     ap.add_argument("--out", required=True)
     ap.add_argument("--iters", type=int, default=8)
+    ap.add_argument("--budget", type=float, default=None, help="stop after this many seconds")
     args = ap.parse_args()
     hip.init(0)
 
     def log(s):
         print(s, file=sys.stderr, flush=True)
 
-    X, T, L, N = build(args.count, args.seed, iters=args.iters, log=log)
+    X, T, L, N = build(args.count, args.seed, iters=args.iters, log=log, budget_s=args.budget)
     np.savez_compressed(args.out, features=X, times=T, lines=L, nnz=N,
                         variants=np.array([tuning.variant_name(i) for i in range(len(tuning.VARIANTS))]))
     log("wrote %s: %d matrices x %d variants" % (args.out, T.shape[0], T.shape[1]))

@@ -1,0 +1,59 @@
+"""Selector pieces on the GPU: device structure statistics against the host
+definitions (the generator's descriptor statistics), the exhaustive
+autotuner, and -- when the committed corpus is present -- that the trained
+network beats the method without it on held-out matrices."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from spmvdare import gen, hip, tuning
+from spmvdare.train_selector import evaluate
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GEN = json.load(open(os.path.join(HERE, "golden", "generator.json")))
+MODELS = os.path.join(os.path.dirname(HERE), "spmv-dare_amd", "spmvdare", "models")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    hip.init(0)
+
+
+@pytest.mark.parametrize("name", ["test_txt", "cfg1", "batch_03", "neigh_two", "tall", "simple_placement"])
+def test_device_stats_match_host(name):
+    a = gen.generate(gen.GenParams.parse(GEN[name]["line"]), stats=True)
+    dm = tuning.DeviceCSR(a, 1)
+    st = dm.stats()
+    host = a.stats
+    assert st["avg_nnz_per_row"] == pytest.approx(host["avg_nnz_per_row"], rel=1e-12)
+    assert st["std_nnz_per_row"] == pytest.approx(host["std_nnz_per_row"], rel=1e-9)
+    assert st["max_nnz_per_row"] == host["max_nnz_per_row"]
+    assert st["avg_bw_scaled"] == pytest.approx(host["avg_bw_scaled"], rel=1e-12)
+    assert st["avg_num_neighbours"] == pytest.approx(host["avg_num_neighbours"], rel=1e-12)
+
+
+def test_autotune_times_every_variant():
+    a = gen.generate(gen.GenParams.parse(GEN["batch_07"]["line"]), stats=False)
+    dm = tuning.DeviceCSR(a, 2)
+    t = tuning.autotune(dm, warmup=1, iters=3)
+    assert t.shape == (40,) and (t > 0).all()
+    f = tuning.features(gen.GenParams.parse(GEN["batch_07"]["line"]), dm.stats())
+    assert f.shape == (14,) and np.isfinite(f).all()
+
+
+def test_trained_selector_beats_static_choice():
+    tr_path = os.path.join(MODELS, "corpus_train.npz")
+    te_path = os.path.join(MODELS, "corpus_test.npz")
+    if not (os.path.exists(tr_path) and os.path.exists(te_path)):
+        pytest.skip("selector corpus not committed yet")
+    tr, te = np.load(tr_path), np.load(te_path)
+    sel = tuning.Selector(seed=1)
+    losses = sel.fit(tr["features"], tr["times"], epochs=150, lr=2e-3)
+    assert losses[-1] < losses[0]
+    res = evaluate(sel, te["features"], te["times"], tr["times"])
+    print(json.dumps(res, indent=1))
+    assert res["mlp_top1"]["mean"] < res["static_best_variant"]["mean"]
+    assert res["mlp_top3_refined"]["mean"] <= 0.05
