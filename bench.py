@@ -49,14 +49,67 @@ def alg_bytes(m, n, nnz):
     return 12 * nnz + 4 * (m + 1) + 8 * n + 8 * m
 
 
-def cpu_baseline(a, x, y_gpu, budget_s):
+REF_CPU_FORMATS = ("csr_naive", "csr_vec", "div_rf")
+
+
+def ref_cpu_exe(fmt):
+    return os.path.join(ROOT, "oracle", "_ref", "cpu_%s_d.exe" % fmt)
+
+
+def cpu_baseline_reference(line, m, n, nnz, cores, timeout_s):
+    """The reference's own CPU formats (csr_naive, csr_vec, div_rf; compiled
+    from its sources with its own driver, src/bench.cpp + src/bench_spmv.cpp)
+    on the host cores: the driver generates the same matrix from the same
+    parameter line, then times >= 64 iterations and >= 2 s and prints the
+    median (bench_spmv.cpp:313-359). sell_c_s has no x86 kernel in the
+    reference (RISC-V vector intrinsics only), so it is not timed here."""
+    import re
+    import subprocess
+    env = dict(os.environ, USE_ARTIFICIAL_MATRICES="1", USE_PROCESSES="0", GPU_KERNEL="0",
+               USE_RCM_REORDERING="0", COOLDOWN="0", NUM_PROCESSES="1", OMP_NUM_THREADS=str(cores),
+               CSRCV_NUM_PACKET_VALS="16384", DIV_VC_TOLERANCE="0", DIV_PACKET_REORDERING="0",
+               DIV_STEALING_ENABLED="1")
+    B = alg_bytes(m, n, nnz)
+    per = {}
+    for fmt in REF_CPU_FORMATS:
+        exe = ref_cpu_exe(fmt)
+        env["PROGG"] = fmt
+        t0 = time.perf_counter()
+        try:
+            r = subprocess.run([exe] + line.split(), env=env, capture_output=True, text=True, timeout=timeout_s)
+        except subprocess.TimeoutExpired:
+            log("cpu baseline %s: no result within %d s" % (fmt, timeout_s))
+            continue
+        mt = re.search(r"time iter: min=([0-9.eE+-]+), median=([0-9.eE+-]+)", r.stdout)
+        ml = re.search(r"number of loops = (\d+)", r.stdout)
+        if r.returncode != 0 or not mt:
+            log("cpu baseline %s: driver exit %d" % (fmt, r.returncode))
+            continue
+        med = float(mt.group(2))
+        per[fmt] = {"gbps": B / med / 1e9, "gflops": 2 * nnz / med / 1e9, "median_ms": med * 1e3,
+                    "loops": int(ml.group(1)) if ml else None, "wall_s": time.perf_counter() - t0}
+        log("cpu baseline %-9s %7.1f GB/s  median %.2f ms  (%d loops, %.0f s with generation)" % (
+            fmt, per[fmt]["gbps"], med * 1e3, per[fmt]["loops"] or 0, per[fmt]["wall_s"]))
+    if not per:
+        return None
+    best = max(per, key=lambda k: per[k]["gbps"])
+    return {"value": round(per[best]["gbps"], 2), "unit": "GB/s", "cores": cores, "kind": "reference",
+            "sample": "reference CPU %s (best of %s; the reference's own driver, >= 64 iterations and >= 2 s, "
+                      "median) on the full matrix, OpenMP on %d host threads, AVX2 build" % (
+                          best, ", ".join(sorted(per)), cores),
+            "format": best, "gflops": round(per[best]["gflops"], 2),
+            "formats": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                        for k, v in per.items()}}
+
+
+def cpu_baseline_port(a, x, y_gpu, budget_s, cores):
     """The oracle's OpenMP csr_naive (equal-row split as csr.cpp:74) on the
     host cores, on the same matrix: one warm-up call, then iterations until
-    the time budget (at least 3), median time."""
+    the time budget (at least 3), median time. Used where the reference's
+    CPU executables were not built."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle
-    cores = len(os.sched_getaffinity(0))
     y, t = pyoracle.csr_naive_omp(a.row_ptr, a.col_ind, a.values, x, threads=cores)
     times = []
     t_end = time.perf_counter() + budget_s
@@ -182,7 +235,8 @@ def main():
     ap.add_argument("--probe", action="store_true", help="measure stream read / copy bandwidth only")
     ap.add_argument("--batched", action="store_true", help="batched workload instead of one matrix")
     ap.add_argument("--count", type=int, default=2048, help="matrices of the batched workload")
-    ap.add_argument("--tile", type=int, default=16, help="units per work item of the batched launch")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="batched launch: units per work item (> 0), or about -tile nnz per item (< 0); 0 = 16384 nnz")
     ap.add_argument("--no-selector", action="store_true")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=128)
@@ -190,7 +244,8 @@ def main():
     ap.add_argument("--workload", default="spmv8m", choices=sorted(WORKLOADS))
     ap.add_argument("--formats", default="all",
                     help="comma list of kind[:sigma], or 'all'")
-    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of the port's timing loop")
+    ap.add_argument("--cpu-timeout", type=int, default=150, help="seconds per reference CPU format run")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
 
@@ -269,9 +324,13 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline ...")
-        cpu, err = cpu_baseline(a, x, y_gpu, args.cpu_budget)
-        log("cpu baseline %.1f GB/s on %d cores; GPU y vs CPU y normwise %.2e" % (cpu["value"], cpu["cores"], err))
+        cores = len(os.sched_getaffinity(0))
+        log("cpu baseline on %d host threads ..." % cores)
+        if all(os.path.exists(ref_cpu_exe(f)) for f in REF_CPU_FORMATS):
+            cpu = cpu_baseline_reference(line, a.m, a.n, a.nnz, cores, args.cpu_timeout)
+        if cpu is None:
+            cpu, err = cpu_baseline_port(a, x, y_gpu, args.cpu_budget, cores)
+            log("cpu baseline (port) %.1f GB/s; GPU y vs CPU y normwise %.2e" % (cpu["value"], err))
 
     out = {
         "metric": "FP64 SpMV GB/s (% HBM peak); learned-selected batched vs plain CSR",

@@ -72,12 +72,16 @@ typedef struct spmvhip_params {
 	int block_nnz;        /* CSR_VEC: nnz per row block staged in LDS (64..1024, default 256) */
 	int split_nnz;        /* CSR_VEC: rows longer than this are split across waves (default 8192) */
 	int xcd_remap;        /* 1 = give each XCD a contiguous range of work (default 0) */
-	int x_window;         /* SELL*: 1 = stage the slice's x window in LDS when it fits */
+	int x_window;         /* SELL*: stage each workgroup's x window [min col, max col] of its
+	                         256 rows in LDS when it spans <= this many doubles (1 = 2048;
+	                         at most 16384); 0 = off */
 	int unroll;           /* SELL*: inner unroll (1, 2, 4, 8; default 8) */
 	int div_threads;      /* DIV_RF: encoder partitions T (default 1) */
 	int div_lanes;        /* DIV_RF: lanes L of the value streams (default 64) */
 	long div_packet_vals; /* DIV_RF: P, max values per packet (default 16384) */
-	int reserved[8];
+	int pipeline;         /* SELL*: 1 = request the next entries' indices and values before the
+	                         current x gathers (software pipelining; default 0) */
+	int reserved[7];
 } spmvhip_params;
 
 typedef struct spmvhip_format spmvhip_format;   /* opaque */
@@ -158,6 +162,8 @@ int spmvhip_time_spmv(spmvhip_format *f, const double *d_x, double *d_y,
  *        5 row blocks (int32, work_items+1) [CSR_VEC]
  *        6 DIV decoded rows (int32, nnz), 7 DIV decoded cols (int32, nnz),
  *        8 DIV decoded values (float64, nnz) -- device decode, packet order [DIV_RF]
+ *        9 x windows (int32 pairs lo, width per workgroup of 4 slices; width 0 =
+ *          not staged)                     [SELL, SELL_C_S with x_window]
  * *bytes_inout: capacity in, bytes written out (or needed, with buf NULL). */
 int spmvhip_export(const spmvhip_format *f, int which, void *buf, size_t *bytes_inout);
 
@@ -166,11 +172,13 @@ void spmvhip_destroy(spmvhip_format *f);
 /* ---- batched SpMV (no counterpart in the reference, whose driver runs one
  * matrix per process, run.sh:116-151) ----
  * Many converted matrices, each with its own device x and y, computed by ONE
- * persistent kernel launch that draws (matrix, unit range) work items from an
- * atomic ticket; with use_graph the launch (ticket reset + kernel) is
- * captured once into a hipGraph and replayed. Members must be CSR_VEC
- * (block_nnz <= 512), SELL or SELL_C_S handles, and must outlive the batch.
- * tile_units: row blocks / slices per work item (0 = 64). */
+ * persistent kernel launch whose wavefronts draw (matrix, unit range) work
+ * items from atomic tickets, one queue per XCD (matrices dealt to the queues
+ * by nnz); with use_graph the launch (ticket reset + kernel) is captured once
+ * into a hipGraph and replayed. Members must be CSR_VEC (block_nnz <= 512),
+ * SELL or SELL_C_S handles, and must outlive the batch.
+ * tile_units > 0: row blocks / slices per work item; 0: items of about 16384
+ * nnz; < 0: items of about -tile_units nnz (at most 2^24). */
 typedef struct spmvhip_batch spmvhip_batch;
 int spmvhip_batch_create(int tile_units, spmvhip_batch **out);
 int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d_x, double *d_y);

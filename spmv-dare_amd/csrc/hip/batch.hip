@@ -1,17 +1,24 @@
 // Batched SpMV: many independent converted matrices, each with its own x and
-// y, computed by ONE persistent kernel launch that takes work from a queue
-// (an atomic ticket over (matrix, unit range) items), optionally replayed
-// from a captured hipGraph. The reference has no counterpart: its driver runs
-// one matrix per process (run.sh:116-151).
+// y, computed by ONE persistent kernel launch that takes work from queues
+// (atomic tickets over (matrix, unit range) items), optionally replayed from
+// a captured hipGraph. The reference has no counterpart: its driver runs one
+// matrix per process (run.sh:116-151).
 //
 // Units are the formats' own per-wave work: a row block of the binned CSR
 // format or a 64-row slice of SELL / SELL-C-sigma (kernels.hpp). A work item
-// is `tile` consecutive units of one matrix; a workgroup (4 waves) takes one
-// item per ticket, its waves striding over the item's units. Long CSR rows
-// are summed by one wave (the split path of the standalone kernel is not
-// used here), so any row length is handled in the single launch.
+// is `tile` consecutive units of one matrix, taken by one wavefront. Long CSR
+// rows are summed by one wave (the split path of the standalone kernel is
+// not used here), so any row length is handled in the single launch.
+//
+// XCD-aware scheduling: the matrices are dealt to 8 queues, one per XCD,
+// balanced by nnz (largest first to the least loaded queue). Workgroup b is
+// dispatched to XCD b % 8 and first drains queue b % 8, so each XCD's L2
+// holds the x vectors of its own matrices only; a wave whose queue is empty
+// moves on to the next queues, which balances the tail. Correctness does not
+// depend on the dispatch order, only locality does.
 
 #include <algorithm>
+#include <numeric>
 #include <vector>
 
 #include "format.hpp"
@@ -19,7 +26,8 @@
 
 namespace spmvhip {
 
-constexpr int kBatchBnnz = 512; // LDS staging per wave of CSR blocks
+constexpr int kBatchBnnzMax = 512; // largest csr_vec block_nnz a member may have
+constexpr long kItemNnz = 16384;   // default work-item size in nnz (one ticket each)
 
 struct BatchMat {
 	int kind, lanes;
@@ -37,35 +45,43 @@ struct WorkItem {
 	int mat, u0, u1, pad;
 };
 
+// zero the kXcds queue counters (a kernel, not a memset: the batch is
+// captured into a hipGraph and replayed)
 __global__ void batch_reset(int *counter)
 {
-	if (threadIdx.x == 0)
-		*counter = 0;
+	if (threadIdx.x < kXcds)
+		counter[threadIdx.x] = 0;
 }
 
+// items[qstart[q] .. qstart[q + 1]) is queue q. BNNZ: LDS staging per wave
+// for CSR row blocks, at least the largest member block_nnz.
+template <int BNNZ>
 __global__ void __launch_bounds__(kBlock) batch_kernel(const BatchMat *__restrict__ mats,
-		const WorkItem *__restrict__ items, int nitems, int *__restrict__ counter)
+		const WorkItem *__restrict__ items, const int *__restrict__ qstart, int *__restrict__ counter)
 {
-	__shared__ double prod[kWavesPerBlock][2 * kBatchBnnz];
-	__shared__ int ticket;
+	__shared__ double prod[kWavesPerBlock][2 * BNNZ];
 	const int wv = threadIdx.x / kWave;
-	for (;;) {
-		if (threadIdx.x == 0)
-			ticket = atomicAdd(counter, 1);
-		__syncthreads();
-		const int it = ticket;
-		__syncthreads(); // every wave has read the ticket before the next one is drawn
-		if (it >= nitems)
-			break; // uniform across the workgroup: every wave leaves together
-		const WorkItem wi = items[it];
-		const BatchMat &M = mats[wi.mat];
-		if (M.kind == SPMVHIP_CSR_VEC) {
-			for (int u = wi.u0 + wv; u < wi.u1; u += kWavesPerBlock)
-				csr_block<kBatchBnnz>(M.rp, M.col, M.val, M.x, M.y, M.blocks[u], M.blocks[u + 1], 0, false,
-						M.lanes, prod[wv]);
-		} else {
-			for (int u = wi.u0 + wv; u < wi.u1; u += kWavesPerBlock)
-				sell_slice<4>(M.sptr, M.sw, M.scol, M.sval, M.order, M.x, M.y, M.m, u, M.lanes);
+	const int home = (int) (blockIdx.x % kXcds);
+	for (int k = 0; k < kXcds; k++) {
+		const int q = (home + k) % kXcds;
+		const int q0 = qstart[q], q1 = qstart[q + 1];
+		for (;;) {
+			int t = 0;
+			if (lane_id() == 0)
+				t = atomicAdd(counter + q, 1);
+			t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, kWave));
+			if (q0 + t >= q1)
+				break; // queue drained: wave-uniform exit, every wave reaches it
+			const WorkItem wi = items[q0 + t];
+			const BatchMat &M = mats[wi.mat];
+			if (M.kind == SPMVHIP_CSR_VEC) {
+				for (int u = wi.u0; u < wi.u1; u++)
+					csr_block<BNNZ>(M.rp, M.col, M.val, M.x, M.y, M.blocks[u], M.blocks[u + 1], 0, false,
+							M.lanes, prod[wv]);
+			} else {
+				for (int u = wi.u0; u < wi.u1; u++)
+					sell_slice<4>(M.sptr, M.sw, M.scol, M.sval, M.order, M.x, M.y, M.m, u, M.lanes);
+			}
 		}
 	}
 }
@@ -73,11 +89,15 @@ __global__ void __launch_bounds__(kBlock) batch_kernel(const BatchMat *__restric
 } // namespace spmvhip
 
 struct spmvhip_batch {
-	int tile = 64;
+	int tile = 0;        // units per work item, or 0: sized by item_nnz
+	long item_nnz = 16384;
+	int bnnz = 256; // LDS staging of the kernel instance (256 or 512)
 	std::vector<spmvhip_format *> fmts;
 	std::vector<spmvhip::BatchMat> host_mats;
 	std::vector<spmvhip::WorkItem> host_items;
-	spmvhip::DevBuf mats, items, counter;
+	std::vector<long> mat_nnz;
+	std::vector<int> host_qstart;
+	spmvhip::DevBuf mats, items, qstart, counter;
 	hipStream_t stream = nullptr;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
@@ -95,8 +115,12 @@ int batch_launch(spmvhip_batch *b, hipStream_t s)
 	batch_reset<<<1, kWave, 0, s>>>(b->counter.as<int>());
 	SPMVHIP_CHECK(hipGetLastError());
 	if (!b->host_items.empty()) {
-		batch_kernel<<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->items.as<WorkItem>(),
-				(int) b->host_items.size(), b->counter.as<int>());
+		if (b->bnnz > 256)
+			batch_kernel<kBatchBnnzMax><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->items.as<WorkItem>(),
+					b->qstart.as<int>(), b->counter.as<int>());
+		else
+			batch_kernel<256><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->items.as<WorkItem>(),
+					b->qstart.as<int>(), b->counter.as<int>());
 		SPMVHIP_CHECK(hipGetLastError());
 	}
 	return SPMVHIP_OK;
@@ -111,12 +135,13 @@ int spmvhip_batch_create(int tile_units, spmvhip_batch **out)
 	if (!out)
 		return set_error(SPMVHIP_EINVAL, "out is NULL");
 	*out = nullptr;
-	if (tile_units < 0 || tile_units > 65536)
+	if (tile_units < -(1 << 24) || tile_units > 65536)
 		return set_error(SPMVHIP_EINVAL, "tile_units %d", tile_units);
 	spmvhip_batch *b = new (std::nothrow) spmvhip_batch();
 	if (!b)
 		return set_error(SPMVHIP_ENOMEM, "batch handle");
-	b->tile = tile_units > 0 ? tile_units : 64;
+	b->tile = tile_units > 0 ? tile_units : 0;
+	b->item_nnz = tile_units < 0 ? -(long) tile_units : kItemNnz;
 	hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
 	if (e != hipSuccess) {
 		delete b;
@@ -139,8 +164,10 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 	M.y = d_y;
 	long units;
 	if (f->kind == SPMVHIP_CSR_VEC) {
-		if (f->block_nnz > kBatchBnnz)
-			return set_error(SPMVHIP_EINVAL, "batch: csr_vec block_nnz %d > %d", f->block_nnz, kBatchBnnz);
+		if (f->block_nnz > kBatchBnnzMax)
+			return set_error(SPMVHIP_EINVAL, "batch: csr_vec block_nnz %d > %d", f->block_nnz, kBatchBnnzMax);
+		if (f->block_nnz > b->bnnz)
+			b->bnnz = kBatchBnnzMax;
 		M.rp = f->row_ptr.as<int32_t>();
 		M.col = f->col.as<int32_t>();
 		M.val = f->val.as<double>();
@@ -163,9 +190,16 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 		return set_error(SPMVHIP_EINVAL, "batch: too many units");
 	int mi = (int) b->host_mats.size();
 	b->host_mats.push_back(M);
+	b->mat_nnz.push_back(f->nnz);
 	b->fmts.push_back(const_cast<spmvhip_format *>(f));
-	for (long u = 0; u < units; u += b->tile)
-		b->host_items.push_back({mi, (int) u, (int) std::min(units, u + b->tile), 0});
+	// units per item: fixed, or enough units of this matrix's average size
+	// for about item_nnz nonzeros (a device-scope ticket costs as much as
+	// streaming several KB, so items are kept coarse)
+	long per = b->tile;
+	if (per <= 0)
+		per = units > 0 ? std::max<long>(1, b->item_nnz * units / std::max<long>(1, f->nnz)) : 1;
+	for (long u = 0; u < units; u += per)
+		b->host_items.push_back({mi, (int) u, (int) std::min(units, u + per), 0});
 	b->total_nnz += f->nnz;
 	return SPMVHIP_OK;
 }
@@ -178,23 +212,52 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 		return set_error(SPMVHIP_EINVAL, "batch already finalized");
 	if (b->host_items.size() > (size_t) INT32_MAX)
 		return set_error(SPMVHIP_EINVAL, "batch: too many work items");
-	SPMVHIP_TRY(b->mats.alloc(std::max<size_t>(1, b->host_mats.size()) * sizeof(BatchMat)));
+	// deal the matrices to the XCD queues: largest first to the least loaded
+	const size_t nm = b->host_mats.size();
+	std::vector<int> by_size(nm), queue_of(nm);
+	std::iota(by_size.begin(), by_size.end(), 0);
+	std::stable_sort(by_size.begin(), by_size.end(), [&](int i, int j) { return b->mat_nnz[i] > b->mat_nnz[j]; });
+	long load[kXcds] = {0};
+	for (int i : by_size) {
+		const int q = (int) (std::min_element(load, load + kXcds) - load);
+		queue_of[i] = q;
+		load[q] += b->mat_nnz[i] + 1;
+	}
+	std::vector<WorkItem> sorted;
+	sorted.reserve(b->host_items.size());
+	b->host_qstart.assign(kXcds + 1, 0);
+	for (int q = 0; q < kXcds; q++) {
+		b->host_qstart[q] = (int) sorted.size();
+		for (const WorkItem &w : b->host_items)
+			if (queue_of[w.mat] == q)
+				sorted.push_back(w);
+	}
+	b->host_qstart[kXcds] = (int) sorted.size();
+	b->host_items.swap(sorted);
+
+	SPMVHIP_TRY(b->mats.alloc(std::max<size_t>(1, nm) * sizeof(BatchMat)));
 	SPMVHIP_TRY(b->items.alloc(std::max<size_t>(1, b->host_items.size()) * sizeof(WorkItem)));
-	SPMVHIP_TRY(b->counter.alloc(sizeof(int)));
-	if (!b->host_mats.empty())
-		SPMVHIP_CHECK(hipMemcpy(b->mats.p, b->host_mats.data(), b->host_mats.size() * sizeof(BatchMat),
-				hipMemcpyHostToDevice));
+	SPMVHIP_TRY(b->qstart.alloc((kXcds + 1) * sizeof(int)));
+	SPMVHIP_TRY(b->counter.alloc(kXcds * sizeof(int)));
+	if (nm)
+		SPMVHIP_CHECK(hipMemcpy(b->mats.p, b->host_mats.data(), nm * sizeof(BatchMat), hipMemcpyHostToDevice));
 	if (!b->host_items.empty())
 		SPMVHIP_CHECK(hipMemcpy(b->items.p, b->host_items.data(), b->host_items.size() * sizeof(WorkItem),
 				hipMemcpyHostToDevice));
-	// persistent grid: as many workgroups as can be resident at once
+	SPMVHIP_CHECK(hipMemcpy(b->qstart.p, b->host_qstart.data(), (kXcds + 1) * sizeof(int), hipMemcpyHostToDevice));
+	// persistent grid: as many workgroups as can be resident at once, a
+	// multiple of the XCD count so that every queue has the same number
 	int dev = 0, per_cu = 0;
 	hipDeviceProp_t prop;
 	SPMVHIP_CHECK(hipGetDevice(&dev));
 	SPMVHIP_CHECK(hipGetDeviceProperties(&prop, dev));
-	SPMVHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel, kBlock, 0));
+	if (b->bnnz > 256)
+		SPMVHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<kBatchBnnzMax>, kBlock, 0));
+	else
+		SPMVHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<256>, kBlock, 0));
 	long g = (long) prop.multiProcessorCount * std::max(per_cu, 1);
-	b->grid = (int) std::max<long>(1, std::min<long>(g, (long) b->host_items.size()));
+	g = std::max<long>(kXcds, g / kXcds * kXcds);
+	b->grid = (int) g;
 	b->finalized = true;
 	if (use_graph) {
 		SPMVHIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));

@@ -495,6 +495,12 @@ int spmvhip_export(const spmvhip_format *f, int which, void *buf, size_t *bytes_
 		if (f->kind != SPMVHIP_DIV_RF)
 			return set_error(SPMVHIP_EINVAL, "decoded streams exist only for DIV_RF");
 		return div_export_decoded(f, which, buf, bytes_inout);
+	case 9:
+		if (!sell || f->x_cap == 0)
+			return set_error(SPMVHIP_EINVAL, "x windows exist only for SELL formats built with x_window");
+		src = f->x_win.p;
+		need = 2 * ((f->nslices + kWavesPerBlock - 1) / kWavesPerBlock) * sizeof(int32_t);
+		break;
 	default: return set_error(SPMVHIP_EINVAL, "unknown export %d", which);
 	}
 	if (!buf) {

@@ -31,6 +31,9 @@ struct spmvhip_format {
 	spmvhip::DevBuf s_cols;       // int32, padded
 	spmvhip::DevBuf s_vals;       // double, padded
 	long nslices = 0, padded = 0;
+	spmvhip::DevBuf x_win;        // int32 (lo, width) per workgroup of 4 slices (x_window only)
+	int x_cap = 0;                // doubles of x staged in LDS per workgroup (0 = no staging)
+	bool x_attr_set = false;      // dynamic-LDS limit raised for the x-window kernel
 
 	// DIV_RF
 	spmvhip::DevBuf div_stream;   // packet bytes

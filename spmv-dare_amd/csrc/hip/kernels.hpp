@@ -122,6 +122,20 @@ __device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const 
 	wave_lds_sync(); // the buffer is reused by the wave's next block
 }
 
+// x[c] for a SELL entry. With XW the workgroup has copied x[lo, lo + W) to
+// xs (LDS): columns in that window are read from LDS, any other column (a
+// padding entry's, or all of them when W = 0) from x.
+template <bool XW>
+__device__ __forceinline__ double gather_x(const double *__restrict__ x, const double *xs, int lo, unsigned W, int c)
+{
+	if (XW) {
+		const unsigned i = (unsigned) (c - lo);
+		if (i < W)
+			return xs[i];
+	}
+	return x[c];
+}
+
 // One SELL slice of 64 rows with T lanes per row (T a power of two; the
 // slice is processed in T rounds of 64 / T rows). Every wave step reads 64
 // consecutive stored entries: with T = 1 these are entry k of all 64 rows
@@ -131,11 +145,16 @@ __device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const 
 // ahead of their FMAs; the T partial sums of a row are added at the end.
 // With ystage != nullptr the results go to ystage[target - row0] (LDS)
 // instead of y[target]: the caller writes the staged rows out coalesced.
-template <int UNROLL>
+//
+// With PIPE the column indices and values of the next UNROLL entries are
+// requested before the x gathers of the current ones, so each step waits for
+// one memory round trip (the gathers) instead of two (indices, then x).
+// With XW, x is read through gather_x from the workgroup's LDS window.
+template <int UNROLL, bool PIPE = false, bool XW = false>
 __device__ __forceinline__ void sell_slice(const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
 		const int32_t *__restrict__ scol, const double *__restrict__ sval, const int32_t *__restrict__ order,
 		const double *__restrict__ x, double *__restrict__ y, long m, long s, int T, double *ystage = nullptr,
-		long row0 = 0)
+		long row0 = 0, const double *xs = nullptr, int lo = 0, unsigned W = 0)
 {
 	constexpr int C = kWave;
 	const int lane = lane_id();
@@ -148,7 +167,38 @@ __device__ __forceinline__ void sell_slice(const int64_t *__restrict__ sptr, con
 		const int32_t *c = scol + off;
 		double acc = 0;
 		int k = 0;
-		for (; k + UNROLL <= steps; k += UNROLL) {
+		if (PIPE && steps >= UNROLL) {
+			double vv[UNROLL];
+			int32_t cc[UNROLL];
+#pragma unroll
+			for (int u = 0; u < UNROLL; u++) {
+				cc[u] = __builtin_nontemporal_load(c + (long) u * C);
+				vv[u] = __builtin_nontemporal_load(v + (long) u * C);
+			}
+			for (k = UNROLL; k + UNROLL <= steps; k += UNROLL) {
+				double xx[UNROLL];
+#pragma unroll
+				for (int u = 0; u < UNROLL; u++)
+					xx[u] = gather_x<XW>(x, xs, lo, W, cc[u]);
+				double vn[UNROLL];
+				int32_t cn[UNROLL];
+#pragma unroll
+				for (int u = 0; u < UNROLL; u++) {
+					cn[u] = __builtin_nontemporal_load(c + (long) (k + u) * C);
+					vn[u] = __builtin_nontemporal_load(v + (long) (k + u) * C);
+				}
+#pragma unroll
+				for (int u = 0; u < UNROLL; u++) {
+					acc = fma(vv[u], xx[u], acc);
+					cc[u] = cn[u];
+					vv[u] = vn[u];
+				}
+			}
+#pragma unroll
+			for (int u = 0; u < UNROLL; u++)
+				acc = fma(vv[u], gather_x<XW>(x, xs, lo, W, cc[u]), acc);
+		}
+		for (; !PIPE && k + UNROLL <= steps; k += UNROLL) {
 			double vv[UNROLL];
 			int32_t cc[UNROLL];
 #pragma unroll
@@ -159,13 +209,13 @@ __device__ __forceinline__ void sell_slice(const int64_t *__restrict__ sptr, con
 			double xx[UNROLL];
 #pragma unroll
 			for (int u = 0; u < UNROLL; u++)
-				xx[u] = x[cc[u]];
+				xx[u] = gather_x<XW>(x, xs, lo, W, cc[u]);
 #pragma unroll
 			for (int u = 0; u < UNROLL; u++)
 				acc = fma(vv[u], xx[u], acc);
 		}
 		for (; k < steps; k++)
-			acc = fma(v[(long) k * C], x[c[(long) k * C]], acc);
+			acc = fma(v[(long) k * C], gather_x<XW>(x, xs, lo, W, c[(long) k * C]), acc);
 		if (T > 1)
 			acc = group_sum(acc, T);
 		const long r = s * C + (long) q * R + lane / T;

@@ -37,6 +37,9 @@ namespace spmvhip {
 constexpr int kSliceC = 64;
 constexpr int kMaxSigma = 16384;
 constexpr int kSortThreads = 256;
+// x window staged in LDS per workgroup (doubles): default and largest
+constexpr int kXWindowDefault = 2048;
+constexpr int kXWindowMax = 16384;
 
 __global__ void row_lengths(const int32_t *__restrict__ rp, long m, int32_t *__restrict__ len)
 {
@@ -195,6 +198,46 @@ __global__ void __launch_bounds__(kBlock) sell_fill(const int32_t *__restrict__ 
 	}
 }
 
+// x window of each workgroup's 256 stored rows (4 slices): the smallest and
+// largest column of their entries. win[2b], win[2b + 1] = (lo, width) when
+// width <= cap, else (0, 0) and workgroup b reads x from global memory.
+__global__ void __launch_bounds__(kBlock) window_ranges(const int32_t *__restrict__ rp,
+		const int32_t *__restrict__ col, const int32_t *__restrict__ order, long m, int cap,
+		int32_t *__restrict__ win)
+{
+	__shared__ int lo_w[kWavesPerBlock], hi_w[kWavesPerBlock];
+	const long r = (long) blockIdx.x * kBlock + threadIdx.x;
+	int lo = INT_MAX, hi = -1;
+	if (r < m) {
+		const long src = order ? order[r] : r;
+		for (int j = rp[src]; j < rp[src + 1]; j++) {
+			const int c = col[j];
+			lo = c < lo ? c : lo;
+			hi = c > hi ? c : hi;
+		}
+	}
+	for (int off = kWave / 2; off > 0; off >>= 1) {
+		const int l2 = __shfl_xor(lo, off, kWave), h2 = __shfl_xor(hi, off, kWave);
+		lo = l2 < lo ? l2 : lo;
+		hi = h2 > hi ? h2 : hi;
+	}
+	if (lane_id() == 0) {
+		lo_w[threadIdx.x / kWave] = lo;
+		hi_w[threadIdx.x / kWave] = hi;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int w = 1; w < kWavesPerBlock; w++) {
+			lo = lo_w[w] < lo ? lo_w[w] : lo;
+			hi = hi_w[w] > hi ? hi_w[w] : hi;
+		}
+		const long width = (long) hi - lo + 1;
+		const bool ok = hi >= 0 && width <= cap;
+		win[2 * blockIdx.x] = ok ? lo : 0;
+		win[2 * blockIdx.x + 1] = ok ? (int) width : 0;
+	}
+}
+
 int sell_convert(spmvhip_format *f, bool sorted)
 {
 	hipStream_t s = f->stream;
@@ -258,6 +301,19 @@ int sell_convert(spmvhip_format *f, bool sorted)
 			f->val.as<double>(), order, sorted ? nullptr : last_ne.as<int64_t>(), f->slice_w.as<int32_t>(),
 			f->slice_ptr.as<int64_t>(), m, f->nslices, T, f->s_cols.as<int32_t>(), f->s_vals.as<double>());
 	SPMVHIP_CHECK(hipGetLastError());
+	f->x_cap = 0;
+	if (f->p.x_window > 0) {
+		const int cap = f->p.x_window == 1 ? kXWindowDefault : f->p.x_window;
+		if (cap > kXWindowMax)
+			return set_error(SPMVHIP_EINVAL, "sell: x_window %d doubles exceeds the LDS limit of %d", cap,
+					kXWindowMax);
+		const long nwg = (f->nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+		SPMVHIP_TRY(f->x_win.alloc(2 * nwg * sizeof(int32_t)));
+		window_ranges<<<(unsigned) nwg, kBlock, 0, s>>>(f->row_ptr.as<int32_t>(), f->col.as<int32_t>(), order, m,
+				cap, f->x_win.as<int32_t>());
+		SPMVHIP_CHECK(hipGetLastError());
+		f->x_cap = cap;
+	}
 	SPMVHIP_CHECK(hipStreamSynchronize(s));
 	// the CSR copy is no longer needed once the slices exist
 	f->row_ptr.release();
@@ -269,17 +325,33 @@ int sell_convert(spmvhip_format *f, bool sorted)
 
 // One wave per slice. UNROLL entries are loaded before their FMAs so that
 // several loads are in flight per lane; the FMAs still run in entry order.
-template <int UNROLL>
+// With XW the workgroup first copies its x window (win, from window_ranges)
+// into dynamic LDS.
+template <int UNROLL, bool PIPE, bool XW>
 __global__ void __launch_bounds__(kBlock) sell_kernel(const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
 		const int32_t *__restrict__ scol, const double *__restrict__ sval, const int32_t *__restrict__ order,
-		const double *__restrict__ x, double *__restrict__ y, long m, long nslices, int T, int remap, int stage)
+		const double *__restrict__ x, double *__restrict__ y, long m, long nslices, int T, int remap, int stage,
+		const int32_t *__restrict__ win)
 {
 	__shared__ double ybuf[kWavesPerBlock * kSliceC];
+	extern __shared__ double xs[];
 	const long blk = xcd_logical_block(remap);
+	const long nwg = (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+	if (blk >= nwg) // padding of an XCD-remapped grid
+		return;
 	const long s = blk * kWavesPerBlock + threadIdx.x / kWave;
+	int lo = 0;
+	unsigned W = 0;
+	if (XW) {
+		lo = win[2 * blk];
+		W = (unsigned) win[2 * blk + 1];
+		for (unsigned i = threadIdx.x; i < W; i += kBlock)
+			xs[i] = x[lo + i];
+		__syncthreads();
+	}
 	if (!stage) {
 		if (s < nslices)
-			sell_slice<UNROLL>(sptr, width, scol, sval, order, x, y, m, s, T);
+			sell_slice<UNROLL, PIPE, XW>(sptr, width, scol, sval, order, x, y, m, s, T, nullptr, 0, xs, lo, W);
 		return;
 	}
 	// The workgroup's 4 slices are rows [row0, row0 + 256) and every target
@@ -287,11 +359,32 @@ __global__ void __launch_bounds__(kBlock) sell_kernel(const int64_t *__restrict_
 	// results in LDS and store them as one contiguous run.
 	const long row0 = blk * kWavesPerBlock * kSliceC;
 	if (s < nslices)
-		sell_slice<UNROLL>(sptr, width, scol, sval, order, x, y, m, s, T, ybuf, row0);
+		sell_slice<UNROLL, PIPE, XW>(sptr, width, scol, sval, order, x, y, m, s, T, ybuf, row0, xs, lo, W);
 	__syncthreads();
 	const long r = row0 + threadIdx.x;
-	if (r < m && row0 < m)
+	if (r < m)
 		y[r] = ybuf[threadIdx.x];
+}
+
+template <int U, bool P>
+static void launch_sell(spmvhip_format *f, unsigned grid, const int32_t *order, const double *x, double *y,
+		int remap, int stage, hipStream_t s)
+{
+	if (f->x_cap > 0) {
+		const size_t lds = (size_t) f->x_cap * sizeof(double);
+		if (!f->x_attr_set) {
+			(void) hipFuncSetAttribute((const void *) sell_kernel<U, P, true>,
+					hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+			f->x_attr_set = true;
+		}
+		sell_kernel<U, P, true><<<grid, kBlock, lds, s>>>(f->slice_ptr.as<int64_t>(), f->slice_w.as<int32_t>(),
+				f->s_cols.as<int32_t>(), f->s_vals.as<double>(), order, x, y, f->m, f->nslices, f->lanes, remap,
+				stage, f->x_win.as<int32_t>());
+	} else {
+		sell_kernel<U, P, false><<<grid, kBlock, 0, s>>>(f->slice_ptr.as<int64_t>(), f->slice_w.as<int32_t>(),
+				f->s_cols.as<int32_t>(), f->s_vals.as<double>(), order, x, y, f->m, f->nslices, f->lanes, remap,
+				stage, nullptr);
+	}
 }
 
 int sell_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
@@ -306,19 +399,19 @@ int sell_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 	// (sigma divides 256) or several lanes per row would store it piecewise
 	const long wg_rows = (long) kWavesPerBlock * kSliceC;
 	const int stage = (order && wg_rows % f->p.sigma == 0) || (!order && f->lanes > 1) ? 1 : 0;
-#define LAUNCH_SELL(U)                                                                                       \
-	sell_kernel<U><<<grid, kBlock, 0, s>>>(f->slice_ptr.as<int64_t>(), f->slice_w.as<int32_t>(),            \
-			f->s_cols.as<int32_t>(), f->s_vals.as<double>(), order, x, y, f->m, f->nslices, f->lanes, remap, \
-			stage)
+	const bool pipe = f->p.pipeline != 0;
 	if (u >= 8)
-		LAUNCH_SELL(8);
+		pipe ? launch_sell<8, true>(f, grid, order, x, y, remap, stage, s)
+		     : launch_sell<8, false>(f, grid, order, x, y, remap, stage, s);
 	else if (u >= 4)
-		LAUNCH_SELL(4);
+		pipe ? launch_sell<4, true>(f, grid, order, x, y, remap, stage, s)
+		     : launch_sell<4, false>(f, grid, order, x, y, remap, stage, s);
 	else if (u >= 2)
-		LAUNCH_SELL(2);
+		pipe ? launch_sell<2, true>(f, grid, order, x, y, remap, stage, s)
+		     : launch_sell<2, false>(f, grid, order, x, y, remap, stage, s);
 	else
-		LAUNCH_SELL(1);
-#undef LAUNCH_SELL
+		pipe ? launch_sell<1, true>(f, grid, order, x, y, remap, stage, s)
+		     : launch_sell<1, false>(f, grid, order, x, y, remap, stage, s);
 	SPMVHIP_CHECK(hipGetLastError());
 	return SPMVHIP_OK;
 }

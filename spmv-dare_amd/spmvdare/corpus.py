@@ -18,12 +18,15 @@ import numpy as np
 from . import gen, hip, tuning
 
 
-def build(count, seed, warmup=2, iters=8, log=None, ranges=None, budget_s=None):
-    params = tuning.sample_params(seed, count * 2, **(ranges or {}))
-    feats, times, lines, nnzs = [], [], [], []
+def build(count, seed, warmup=2, iters=8, log=None, ranges=None, budget_s=None, start=0):
+    """Time `count` matrices drawn from sample_params(seed), beginning at draw
+    `start` (so a corpus can be built in several sessions and concatenated)."""
+    params = tuning.sample_params(seed, start + count * 2, **(ranges or {}))
+    feats, times, lines, nnzs, pidx = [], [], [], [], []
     t0 = time.time()
     k = 0
-    for p in params:
+    for i in range(start, len(params)):
+        p = params[i]
         if len(lines) == count:
             break
         if budget_s is not None and time.time() - t0 > budget_s:
@@ -40,11 +43,12 @@ def build(count, seed, warmup=2, iters=8, log=None, ranges=None, budget_s=None):
         times.append(tuning.autotune(dm, warmup=warmup, iters=iters))
         lines.append(p.line())
         nnzs.append(a.nnz)
+        pidx.append(i)
         del dm
         k += 1
         if log and (k % 32 == 0 or k == count):
             log("corpus %d/%d matrices, %.0f s" % (k, count, time.time() - t0))
-    return np.array(feats), np.array(times), np.array(lines), np.array(nnzs)
+    return np.array(feats), np.array(times), np.array(lines), np.array(nnzs), np.array(pidx)
 
 
 def main():
@@ -54,14 +58,16 @@ def main():
     ap.add_argument("--out", required=True)
     ap.add_argument("--iters", type=int, default=8)
     ap.add_argument("--budget", type=float, default=None, help="stop after this many seconds")
+    ap.add_argument("--start", type=int, default=0, help="first parameter draw to use")
     args = ap.parse_args()
     hip.init(0)
 
     def log(s):
         print(s, file=sys.stderr, flush=True)
 
-    X, T, L, N = build(args.count, args.seed, iters=args.iters, log=log, budget_s=args.budget)
-    np.savez_compressed(args.out, features=X, times=T, lines=L, nnz=N,
+    X, T, L, N, I = build(args.count, args.seed, iters=args.iters, log=log, budget_s=args.budget,
+                          start=args.start)
+    np.savez_compressed(args.out, features=X, times=T, lines=L, nnz=N, draw=I,
                         variants=np.array([tuning.variant_name(i) for i in range(len(tuning.VARIANTS))]))
     log("wrote %s: %d matrices x %d variants" % (args.out, T.shape[0], T.shape[1]))
 

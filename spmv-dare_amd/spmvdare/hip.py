@@ -29,7 +29,8 @@ class Params(ctypes.Structure):
                 ("split_nnz", ctypes.c_int), ("xcd_remap", ctypes.c_int),
                 ("x_window", ctypes.c_int), ("unroll", ctypes.c_int),
                 ("div_threads", ctypes.c_int), ("div_lanes", ctypes.c_int),
-                ("div_packet_vals", ctypes.c_long), ("reserved", ctypes.c_int * 8)]
+                ("div_packet_vals", ctypes.c_long), ("pipeline", ctypes.c_int),
+                ("reserved", ctypes.c_int * 7)]
 
 
 class Info(ctypes.Structure):

@@ -72,6 +72,28 @@ def test_batch_matches_members(members, use_graph, tile):
     b.destroy()
 
 
+def test_batch_with_block512_member(members):
+    """A member with block_nnz = 512 switches the batch to the kernel with the
+    larger LDS staging; every member must still be right."""
+    a = gen.generate(gen.GenParams.parse(GEN["cfg1"]["line"]), stats=False)
+    x = gen.make_x(a.n, 77)
+    f = hip.csr_to_format(a.row_ptr, a.col_ind, a.values, a.m, a.n, a.nnz, kind="csr_vec", block_nnz=512)
+    big = dict(name="cfg1_b512", kind="csr_vec", fmt=f, dx=hip.DeviceArray.from_host(x), dy=hip.DeviceArray(a.m),
+               y_ref=pyoracle.csr_naive(a.row_ptr, a.col_ind, a.values, x), m=a.m)
+    group = members[:7] + [big]
+    b = hip.Batch(4)
+    for mb in group:
+        b.add(mb["fmt"], mb["dx"], mb["dy"])
+    b.finalize(use_graph=True)
+    for rep in range(2):
+        for mb in group:
+            mb["dy"].upload(np.full(mb["m"], np.nan))
+        b.run()
+        b.synchronize()
+        check(group)
+    b.destroy()
+
+
 def test_batch_rejects_unbatchable(members):
     a = gen.generate(gen.GenParams.parse(GEN["test_txt"]["line"]), stats=False)
     b = hip.Batch()

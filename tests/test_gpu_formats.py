@@ -220,11 +220,78 @@ def test_sell_bit_exact(name, unroll):
     np.testing.assert_array_equal(y, pyoracle.sell_spmv(ref, 64, x, m))
 
 
+def banded(m, half, seed):
+    """Rows of 0..40 columns inside [r - half, r + half]."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for r in range(m):
+        lo, hi = max(0, r - half), min(m, r + half + 1)
+        k = min(hi - lo, int(rng.integers(0, 41)))
+        rows.append(np.sort(rng.choice(np.arange(lo, hi), size=k, replace=False)))
+    rp = np.zeros(m + 1, np.int32)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    ci = np.concatenate(rows).astype(np.int32)
+    return rp, ci, rng.uniform(-1, 1, ci.size), m
+
+
+XW_CASES = ["banded_300", "banded_2500", "block_edges", "ragged_4097", "all_empty", "one_row_empty", "cfg1"]
+
+
+def xw_matrix(name):
+    if name == "banded_300":
+        return banded(3001, 300, 11)
+    if name == "banded_2500":
+        return banded(9000, 2500, 12)
+    return matrix(name)
+
+
+@pytest.mark.parametrize("name", XW_CASES)
+@pytest.mark.parametrize("kind,sigma,T,cap", [("sell_c_s", 256, 1, 1), ("sell_c_s", 4096, 1, 16384),
+                                              ("sell_c_s", 64, 4, 1), ("sell", 1, 1, 8192), ("sell", 1, 2, 1)])
+@pytest.mark.parametrize("pipeline", [0, 1])
+def test_sell_x_window_and_pipeline(name, kind, sigma, T, cap, pipeline):
+    """x read from a per-workgroup LDS window (columns outside it, i.e. padding
+    entries, from global memory) and the software-pipelined loads keep every
+    FMA in entry order: y is bit-identical to the same layout without them.
+    The windows are checked against their definition (min / max column of the
+    workgroup's 256 stored rows, staged only when the span fits)."""
+    rp, ci, va, n = xw_matrix(name)
+    m = rp.size - 1
+    print("case", name, kind, sigma, T, cap, pipeline, flush=True)
+    x = gen.make_x(n, 93)
+    kw = dict(lanes_per_row=T, unroll=4)
+    if kind == "sell_c_s":
+        kw["sigma"] = sigma
+    plain = hip.csr_to_format(rp, ci, va, m, n, ci.size, kind=kind, **kw)
+    f = hip.csr_to_format(rp, ci, va, m, n, ci.size, kind=kind, x_window=cap, pipeline=pipeline, **kw)
+    y0 = run(plain, x, m)
+    y = run(f, x, m)
+    np.testing.assert_array_equal(y, y0)
+    if m == 0:
+        return
+    win = f.export(9, np.int32).reshape(-1, 2)
+    order = f.export(0, np.int32) if kind == "sell_c_s" else np.arange(m, dtype=np.int32)
+    limit = 2048 if cap == 1 else cap
+    staged = 0
+    for b in range(win.shape[0]):
+        rows = order[b * 256:(b + 1) * 256]
+        cols = np.concatenate([ci[rp[r]:rp[r + 1]] for r in rows]) if rows.size else np.zeros(0, np.int32)
+        if cols.size and cols.max() - cols.min() + 1 <= limit:
+            assert tuple(win[b]) == (cols.min(), cols.max() - cols.min() + 1)
+            staged += 1
+        else:
+            assert tuple(win[b]) == (0, 0)
+    if name.startswith("banded") and limit >= 2 * int(name.split("_")[1]) + 256:
+        assert staged > 0  # the band plus a workgroup's rows fits: windows must be staged
+    check_dd(y, rp, ci, va, x, "%s xw%d pipe%d/%s" % (kind, cap, pipeline, name))
+
+
 def test_bad_parameters_are_errors():
     rp, ci, va, n = matrix("test_txt")
     m = rp.size - 1
     for kind, kw in [("sell_c_s", dict(sigma=20000)), ("sell", dict(C=32)),
-                     ("csr_vec", dict(block_nnz=8)), ("csr_vec", dict(lanes_per_row=3))]:
+                     ("csr_vec", dict(block_nnz=8)), ("csr_vec", dict(lanes_per_row=3)),
+                     ("sell_c_s", dict(x_window=20000))]:
         with pytest.raises(hip.SpmvHipError) as e:
             hip.csr_to_format(rp, ci, va, m, n, ci.size, kind=kind, **kw)
         assert e.value.code == -1
