@@ -81,7 +81,11 @@ typedef struct spmvhip_params {
 	long div_packet_vals; /* DIV_RF: P, max values per packet (default 16384) */
 	int pipeline;         /* SELL*: 1 = request the next entries' indices and values before the
 	                         current x gathers (software pipelining; default 0) */
-	int reserved[7];
+	int y_mode;           /* SELL*: how results reach y. 0 auto; 1 direct stores (y[row_order[i]]);
+	                         2 staged in LDS per workgroup of 256 rows (needs sigma | 256 or no
+	                         sort); 3 stored in sorted order to a device buffer, then one
+	                         coalesced permutation pass y[r] = ys[inv[r]] (SELL_C_S) */
+	int reserved[6];
 } spmvhip_params;
 
 typedef struct spmvhip_format spmvhip_format;   /* opaque */

@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(kBlock) csr_binned_kernel(const int32_t *__res
 		const int32_t *__restrict__ blocks, long nblocks, int split_nnz, int lanes_cfg, int remap)
 {
 	__shared__ double prod[kWavesPerBlock][2 * BNNZ_MAX];
-	const int wv = threadIdx.x / kWave;
+	const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); // wave-uniform: scalar loads below
 	const long w = xcd_logical_block(remap) * kWavesPerBlock + wv;
 	if (w >= nblocks)
 		return; // waves never synchronise with each other: early exit is safe

@@ -26,6 +26,8 @@ struct spmvhip_format {
 
 	// SELL / SELL_C_S
 	spmvhip::DevBuf row_order;    // int32, m (SELL_C_S only)
+	spmvhip::DevBuf row_inv;      // int32, m: sorted position of each row (SELL_C_S, y_mode 3)
+	spmvhip::DevBuf ysorted;      // double, nslices * 64: results in sorted order (SELL_C_S, y_mode 3)
 	spmvhip::DevBuf slice_w;      // int32, nslices
 	spmvhip::DevBuf slice_ptr;    // int64, nslices + 1
 	spmvhip::DevBuf s_cols;       // int32, padded

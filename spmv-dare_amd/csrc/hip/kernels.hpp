@@ -63,13 +63,24 @@ __device__ __forceinline__ double wave_row_sum(const int32_t *__restrict__ col, 
 // by the calling wavefront. `p` is the wave's LDS buffer of 2 * BNNZ_MAX
 // doubles. A single row longer than BNNZ_MAX is summed by the whole wave;
 // with skip_split, rows longer than split_nnz are left to the split kernels.
+//
+// The block bounds are wave-uniform (scalar registers), so the control below
+// runs on the scalar unit. Lane i loads the bounds of row r0 + i before the
+// stream phase, so the reduction does not wait for a second round of
+// row-pointer loads. The stream phase is branch-free: 4 x 64 entries are
+// requested per step, lanes past the block's end re-read its last entry and
+// store a zero product, so a block of <= 256 nonzeros costs one index round
+// trip and one gather round trip.
 template <int BNNZ_MAX>
 __device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
 		const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y, int r0, int r1,
 		int split_nnz, bool skip_split, int lanes_cfg, double *p)
 {
+	static_assert(BNNZ_MAX % 128 == 0, "the stream phase writes p in steps of 256 entries");
 	const int lane = lane_id();
-	const int j0 = rp[r0], j1 = rp[r1];
+	r0 = __builtin_amdgcn_readfirstlane(r0);
+	r1 = __builtin_amdgcn_readfirstlane(r1);
+	const int j0 = __builtin_amdgcn_readfirstlane(rp[r0]), j1 = __builtin_amdgcn_readfirstlane(rp[r1]);
 	const int nrows = r1 - r0, nz = j1 - j0;
 	if (nrows == 1 && nz > BNNZ_MAX) {
 		if (skip_split && nz > split_nnz)
@@ -79,42 +90,50 @@ __device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const 
 			y[r0] = sum;
 		return;
 	}
-	// stream block: products to LDS, coalesced over the block's nnz, four
-	// loads per lane in flight
-	int e = lane;
-	for (; e + 3 * kWave < nz; e += 4 * kWave) {
+	// start of row r0 + lane relative to the block start (nrows <= 64; lanes
+	// past the block read its end): one unconditional load, in flight during
+	// the stream phase; row r ends where row r + 1 starts
+	const int ra = rp[r0 + (lane < nrows ? lane : nrows)] - j0;
+	// stream phase: products to LDS, coalesced over the block's nnz (nz < 2 *
+	// BNNZ_MAX for a multi-row block, so p[e] stays inside the buffer)
+	for (int e0 = 0; e0 < nz; e0 += 4 * kWave) {
 		double v[4];
 		int32_t c[4];
 #pragma unroll
 		for (int u = 0; u < 4; u++) {
-			v[u] = __builtin_nontemporal_load(val + j0 + e + u * kWave);
-			c[u] = __builtin_nontemporal_load(col + j0 + e + u * kWave);
+			const int e = e0 + u * kWave + lane;
+			const int ec = e < nz ? e : nz - 1;
+			v[u] = __builtin_nontemporal_load(val + j0 + ec);
+			c[u] = __builtin_nontemporal_load(col + j0 + ec);
 		}
 		double xv[4];
 #pragma unroll
 		for (int u = 0; u < 4; u++)
 			xv[u] = x[c[u]];
 #pragma unroll
-		for (int u = 0; u < 4; u++)
-			p[e + u * kWave] = v[u] * xv[u];
+		for (int u = 0; u < 4; u++) {
+			const int e = e0 + u * kWave + lane;
+			p[e] = e < nz ? v[u] * xv[u] : 0.0;
+		}
 	}
-	for (; e < nz; e += kWave)
-		p[e] = val[j0 + e] * x[col[j0 + e]];
 	wave_lds_sync();
-	// lanes per row: largest power of two <= 64 / nrows (or the configured value)
-	int L = kWave;
-	while (L > 1 && L * nrows > kWave)
-		L >>= 1;
-	if (lanes_cfg > 0 && lanes_cfg < L)
-		L = lanes_cfg;
-	const int g = lane / L, sub = lane - g * L, ngroups = kWave / L;
-	for (int r = g; r < nrows + ((ngroups - (nrows % ngroups)) % ngroups); r += ngroups) {
+	// lanes per row L = 2^lg: largest power of two <= 64 / nrows (or the
+	// configured value); groups of L lanes take rows g, g + 64/L, ...
+	int lg = 6;
+	while (lg > 0 && (nrows << lg) > kWave)
+		lg--;
+	if (lanes_cfg > 0 && lanes_cfg < (1 << lg))
+		lg = 31 - __builtin_clz(lanes_cfg);
+	const int L = 1 << lg, g = lane >> lg, sub = lane & (L - 1), gshift = 6 - lg;
+	const int rows_up = (nrows + (1 << gshift) - 1) >> gshift << gshift;
+	for (int r = g; r < rows_up; r += 1 << gshift) {
+		const int a = __shfl(ra, r & (kWave - 1), kWave);
+		const int bn = __shfl(ra, (r + 1) & (kWave - 1), kWave);
+		const int b = r + 1 < kWave ? bn : nz;
 		double s = 0;
-		if (r < nrows) {
-			const int a = rp[r0 + r] - j0, b = rp[r0 + r + 1] - j0;
+		if (r < nrows)
 			for (int e = a + sub; e < b; e += L)
 				s += p[e];
-		}
 		s = group_sum(s, L);
 		if (r < nrows && sub == 0)
 			y[r0 + r] = s;
@@ -134,6 +153,29 @@ __device__ __forceinline__ double gather_x(const double *__restrict__ x, const d
 			return xs[i];
 	}
 	return x[c];
+}
+
+// N consecutive wave steps k..k+N-1 of a SELL slice (v, c point at the
+// lane's entry of step 0): all loads issued before the gathers, the FMAs in
+// entry order.
+template <int N, bool XW>
+__device__ __forceinline__ double sell_group(const double *__restrict__ v, const int32_t *__restrict__ c,
+		const double *__restrict__ x, const double *xs, int lo, unsigned W, int k, double acc)
+{
+	double vv[N], xx[N];
+	int32_t cc[N];
+#pragma unroll
+	for (int u = 0; u < N; u++) {
+		vv[u] = __builtin_nontemporal_load(v + (long) (k + u) * kWave);
+		cc[u] = __builtin_nontemporal_load(c + (long) (k + u) * kWave);
+	}
+#pragma unroll
+	for (int u = 0; u < N; u++)
+		xx[u] = gather_x<XW>(x, xs, lo, W, cc[u]);
+#pragma unroll
+	for (int u = 0; u < N; u++)
+		acc = fma(vv[u], xx[u], acc);
+	return acc;
 }
 
 // One SELL slice of 64 rows with T lanes per row (T a power of two; the
@@ -214,8 +256,19 @@ __device__ __forceinline__ void sell_slice(const int64_t *__restrict__ sptr, con
 			for (int u = 0; u < UNROLL; u++)
 				acc = fma(vv[u], xx[u], acc);
 		}
-		for (; k < steps; k++)
-			acc = fma(v[(long) k * C], gather_x<XW>(x, xs, lo, W, c[(long) k * C]), acc);
+		// the last steps - k < UNROLL entries in groups of UNROLL/2, UNROLL/4,
+		// ..., 1 (steps is uniform over the wave): each group's loads are in
+		// flight together
+		if (UNROLL >= 8 && k + 4 <= steps) {
+			acc = sell_group<4, XW>(v, c, x, xs, lo, W, k, acc);
+			k += 4;
+		}
+		if (UNROLL >= 4 && k + 2 <= steps) {
+			acc = sell_group<2, XW>(v, c, x, xs, lo, W, k, acc);
+			k += 2;
+		}
+		if (k < steps)
+			acc = sell_group<1, XW>(v, c, x, xs, lo, W, k, acc);
 		if (T > 1)
 			acc = group_sum(acc, T);
 		const long r = s * C + (long) q * R + lane / T;

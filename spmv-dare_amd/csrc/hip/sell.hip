@@ -198,6 +198,24 @@ __global__ void __launch_bounds__(kBlock) sell_fill(const int32_t *__restrict__ 
 	}
 }
 
+// inv[order[i]] = i: the sorted position of every row
+__global__ void invert_order(const int32_t *__restrict__ order, long m, int32_t *__restrict__ inv)
+{
+	const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < m)
+		inv[order[i]] = (int32_t) i;
+}
+
+// y[r] = ys[inv[r]]: results stored in sorted order back to row order (the
+// reads stay inside each row's sigma window, so they mostly hit L2)
+__global__ void unpermute_y(const double *__restrict__ ys, const int32_t *__restrict__ inv, long m,
+		double *__restrict__ y)
+{
+	const long r = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	if (r < m)
+		y[r] = ys[inv[r]];
+}
+
 // x window of each workgroup's 256 stored rows (4 slices): the smallest and
 // largest column of their entries. win[2b], win[2b + 1] = (lo, width) when
 // width <= cap, else (0, 0) and workgroup b reads x from global memory.
@@ -255,6 +273,11 @@ int sell_convert(spmvhip_format *f, bool sorted)
 	f->p.C = kSliceC;
 	f->p.sigma = sorted ? sigma : 1;
 	f->nslices = (m + kSliceC - 1) / kSliceC;
+	const int ym = f->p.y_mode;
+	if (ym < 0 || ym > 3)
+		return set_error(SPMVHIP_EINVAL, "sell: y_mode %d (0 auto, 1 direct, 2 LDS-staged, 3 sorted buffer)", ym);
+	if (ym == 2 && sorted && (kWavesPerBlock * kSliceC) % sigma != 0)
+		return set_error(SPMVHIP_EINVAL, "sell_c_s: y_mode 2 needs sigma dividing 256 (got %ld)", sigma);
 
 	DevBuf len, sizes;
 	SPMVHIP_TRY(len.alloc((m + 1) * sizeof(int32_t)));
@@ -273,6 +296,12 @@ int sell_convert(spmvhip_format *f, bool sorted)
 				f->row_order.as<int32_t>());
 		SPMVHIP_CHECK(hipGetLastError());
 		order = f->row_order.as<int32_t>();
+		if (ym == 3) {
+			SPMVHIP_TRY(f->row_inv.alloc(m * sizeof(int32_t)));
+			SPMVHIP_TRY(f->ysorted.alloc(f->nslices * kSliceC * sizeof(double)));
+			invert_order<<<grid_for(m, kBlock), kBlock, 0, s>>>(order, m, f->row_inv.as<int32_t>());
+			SPMVHIP_CHECK(hipGetLastError());
+		}
 	}
 	SPMVHIP_TRY(f->slice_w.alloc(f->nslices * sizeof(int32_t)));
 	SPMVHIP_TRY(f->slice_ptr.alloc((f->nslices + 1) * sizeof(int64_t)));
@@ -339,7 +368,7 @@ __global__ void __launch_bounds__(kBlock) sell_kernel(const int64_t *__restrict_
 	const long nwg = (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
 	if (blk >= nwg) // padding of an XCD-remapped grid
 		return;
-	const long s = blk * kWavesPerBlock + threadIdx.x / kWave;
+	const long s = blk * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
 	int lo = 0;
 	unsigned W = 0;
 	if (XW) {
@@ -395,24 +424,42 @@ int sell_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 	const unsigned grid = xcd_padded_grid(grid_for(f->nslices, kWavesPerBlock), remap);
 	const int32_t *order = f->kind == SPMVHIP_SELL_C_S ? f->row_order.as<int32_t>() : nullptr;
 	int u = f->p.unroll > 0 ? f->p.unroll : 8;
-	// y through LDS when the permutation cannot leave a workgroup's 256 rows
-	// (sigma divides 256) or several lanes per row would store it piecewise
+	// y_mode 0 (auto): through LDS when the permutation cannot leave a
+	// workgroup's 256 rows (sigma divides 256) or several lanes per row would
+	// store it piecewise, else direct stores
 	const long wg_rows = (long) kWavesPerBlock * kSliceC;
-	const int stage = (order && wg_rows % f->p.sigma == 0) || (!order && f->lanes > 1) ? 1 : 0;
+	int stage = 0;
+	double *yk = y;
+	switch (f->p.y_mode) {
+	case 0: stage = (order && wg_rows % f->p.sigma == 0) || (!order && f->lanes > 1) ? 1 : 0; break;
+	case 1: stage = 0; break;
+	case 2: stage = 1; break;
+	case 3:
+		if (order) { // results in sorted order, no permutation inside the kernel
+			yk = f->ysorted.as<double>();
+			order = nullptr;
+		}
+		stage = 0;
+		break;
+	}
 	const bool pipe = f->p.pipeline != 0;
 	if (u >= 8)
-		pipe ? launch_sell<8, true>(f, grid, order, x, y, remap, stage, s)
-		     : launch_sell<8, false>(f, grid, order, x, y, remap, stage, s);
+		pipe ? launch_sell<8, true>(f, grid, order, x, yk, remap, stage, s)
+		     : launch_sell<8, false>(f, grid, order, x, yk, remap, stage, s);
 	else if (u >= 4)
-		pipe ? launch_sell<4, true>(f, grid, order, x, y, remap, stage, s)
-		     : launch_sell<4, false>(f, grid, order, x, y, remap, stage, s);
+		pipe ? launch_sell<4, true>(f, grid, order, x, yk, remap, stage, s)
+		     : launch_sell<4, false>(f, grid, order, x, yk, remap, stage, s);
 	else if (u >= 2)
-		pipe ? launch_sell<2, true>(f, grid, order, x, y, remap, stage, s)
-		     : launch_sell<2, false>(f, grid, order, x, y, remap, stage, s);
+		pipe ? launch_sell<2, true>(f, grid, order, x, yk, remap, stage, s)
+		     : launch_sell<2, false>(f, grid, order, x, yk, remap, stage, s);
 	else
-		pipe ? launch_sell<1, true>(f, grid, order, x, y, remap, stage, s)
-		     : launch_sell<1, false>(f, grid, order, x, y, remap, stage, s);
+		pipe ? launch_sell<1, true>(f, grid, order, x, yk, remap, stage, s)
+		     : launch_sell<1, false>(f, grid, order, x, yk, remap, stage, s);
 	SPMVHIP_CHECK(hipGetLastError());
+	if (yk != y) {
+		unpermute_y<<<grid_for(f->m, kBlock), kBlock, 0, s>>>(yk, f->row_inv.as<int32_t>(), f->m, y);
+		SPMVHIP_CHECK(hipGetLastError());
+	}
 	return SPMVHIP_OK;
 }
 

@@ -14,7 +14,7 @@
 //                     and when the object is destroyed
 //   SELLCS_C, SELLCS_SIGMA          SELL-C-sigma slice height / sort window
 //   CSRV_BLOCK_NNZ, CSRV_LANES, CSRV_SPLIT_NNZ   CSR-vector row-block tuning
-//   SPMV_XCD_REMAP, SELL_UNROLL, SELL_PIPELINE, SELL_X_WINDOW
+//   SPMV_XCD_REMAP, SELL_UNROLL, SELL_PIPELINE, SELL_X_WINDOW, SELL_Y_MODE
 //   DIV_THREADS, DIV_LANES, CSRCV_NUM_PACKET_VALS   DIV-RF codec parameters
 
 #include <cstdio>
@@ -111,6 +111,7 @@ struct Matrix_Format *csr_to_format(INT_T *row_ptr, INT_T *col_ind, ValueTypeRef
 	p.unroll = (int) env_long("SELL_UNROLL", 0);
 	p.pipeline = (int) env_long("SELL_PIPELINE", 0);
 	p.x_window = (int) env_long("SELL_X_WINDOW", 0);
+	p.y_mode = (int) env_long("SELL_Y_MODE", 0);
 	p.div_threads = (int) env_long("DIV_THREADS", 0);
 	p.div_lanes = (int) env_long("DIV_LANES", 0);
 	p.div_packet_vals = env_long("CSRCV_NUM_PACKET_VALS", 0);

@@ -286,6 +286,31 @@ def test_sell_x_window_and_pipeline(name, kind, sigma, T, cap, pipeline):
     check_dd(y, rp, ci, va, x, "%s xw%d pipe%d/%s" % (kind, cap, pipeline, name))
 
 
+@pytest.mark.parametrize("name", ["cfg1", "ragged_4097", "huge_rows", "block_edges", "one_row_empty", "all_empty",
+                                  "dense_rows"])
+@pytest.mark.parametrize("kind,sigma,T", [("sell_c_s", 64, 1), ("sell_c_s", 256, 4), ("sell_c_s", 4096, 1),
+                                          ("sell_c_s", 4096, 8), ("sell", 1, 1), ("sell", 1, 4)])
+@pytest.mark.parametrize("y_mode", [1, 2, 3])
+def test_sell_y_modes(name, kind, sigma, T, y_mode):
+    """Direct stores, LDS staging and the sorted buffer + permutation pass only
+    change where each result is stored: y is bit-identical to the default."""
+    rp, ci, va, n = matrix(name)
+    m = rp.size - 1
+    print("case", name, kind, sigma, T, y_mode, flush=True)
+    x = gen.make_x(n, 92)
+    kw = dict(lanes_per_row=T, unroll=4)
+    if kind == "sell_c_s":
+        kw["sigma"] = sigma
+    if y_mode == 2 and kind == "sell_c_s" and 256 % sigma:
+        with pytest.raises(hip.SpmvHipError):
+            hip.csr_to_format(rp, ci, va, m, n, ci.size, kind=kind, y_mode=y_mode, **kw)
+        return
+    y0 = run(hip.csr_to_format(rp, ci, va, m, n, ci.size, kind=kind, **kw), x, m)
+    y = run(hip.csr_to_format(rp, ci, va, m, n, ci.size, kind=kind, y_mode=y_mode, **kw), x, m)
+    np.testing.assert_array_equal(y, y0)
+    assert (y[np.diff(rp) == 0] == 0).all()
+
+
 def test_bad_parameters_are_errors():
     rp, ci, va, n = matrix("test_txt")
     m = rp.size - 1
