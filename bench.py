@@ -30,11 +30,13 @@ WORKLOADS = {
                  "random, bw 0.3, 1 neighbour, cross-row 0.2, seed 14"),
 }
 CANDIDATES = [
-    ("csr_vec", {}),
-    ("sell_c_s", {"sigma": 4096}),
+    ("csr_vec", {"block_nnz": 256}),
+    ("sell_c_s", {"sigma": 256, "lanes_per_row": 4, "unroll": 4}),
     ("sell_c_s", {"sigma": 256}),
+    ("sell_c_s", {"sigma": 4096, "lanes_per_row": 4, "unroll": 4}),
     ("sell_c_s", {"sigma": 1}),
-    ("sell", {}),
+    ("sell", {"lanes_per_row": 4, "unroll": 4}),
+    ("div_rf", {"div_packet_vals": 1024, "div_threads": 64, "unroll": 8}),
     ("csr_naive", {}),
 ]
 
@@ -126,6 +128,29 @@ def cpu_baseline_port(a, x, y_gpu, budget_s, cores):
             "gflops": 2 * a.nnz / med / 1e9}, err
 
 
+PROFILED = {  # the variant each profile summary's byte counters were taken on
+    ("spmv8m", "csr_vec"): {"block_nnz": 256},
+    ("spmv8m", "sell_c_s"): {"sigma": 256, "lanes_per_row": 4, "unroll": 4},
+    ("spmv8m", "div_rf"): {"div_packet_vals": 1024, "div_threads": 64, "unroll": 8},
+}
+
+
+def profiled_traffic(workload, fmt, params):
+    """HBM bytes per launch of this format on this workload from the committed
+    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE, calibrated), or None when
+    the summary does not cover exactly this variant."""
+    import glob
+    import re
+    if PROFILED.get((workload, fmt)) != params:
+        return None
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "%s_*.md" % workload))):
+        for line in open(path):
+            m = re.match(r"^\| %s \| [^|]+ \| \d+ \| \d+ \| (\d+) \| [\d.]+ \|\s*$" % re.escape(fmt), line)
+            if m:
+                return int(m.group(1))
+    return None
+
+
 def parse_format(tok):
     """kind[:sigma] or kind:key=value;key=value"""
     k, _, rest = tok.partition(":")
@@ -164,7 +189,11 @@ def run_batched(args, gen, hip, np):
     model = os.path.join(ROOT, "spmv-dare_amd", "spmvdare", "models", "selector.json")
     sel = tuning.Selector.load(model) if os.path.exists(model) and not args.no_selector else None
     fixed = tuning.VARIANTS.index(("sell_c_s", {"sigma": 256, "lanes_per_row": 4, "unroll": 4}))
-    csr = tuning.VARIANTS.index(("csr_vec", {"block_nnz": 512, "lanes_per_row": 0}))
+    csr = tuning.VARIANTS.index(("csr_vec", {"block_nnz": 256, "lanes_per_row": 0}))  # the default csr_vec
+    # the persistent launch stages CSR row blocks of at most 256 nnz at full
+    # occupancy (a 512-nnz member would halve it for every member): the
+    # selector chooses among the variants that keep it
+    allowed = np.array([not (k == "csr_vec" and kw.get("block_nnz", 256) > 256) for k, kw in tuning.VARIANTS])
     params = tuning.sample_params(21, count * 2)
     t0 = time.perf_counter()
     members, total_nnz, total_bytes, chosen = [], 0, 0, []
@@ -177,7 +206,7 @@ def run_batched(args, gen, hip, np):
             continue
         dm = tuning.DeviceCSR(a, p.seed + 1)
         if sel is not None:
-            v = int(sel.select(tuning.features(p, dm.stats()))[0, 0])
+            v = int(sel.select(tuning.features(p, dm.stats()), allowed=allowed)[0, 0])
         else:
             v = fixed
         chosen.append(v)
@@ -317,10 +346,7 @@ def main():
     hip.synchronize()
     y_gpu = dy.download()
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (args.workload, r["format"]))
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+    traffic = profiled_traffic(args.workload, r["format"], r["params"])
 
     cpu = None
     if not args.no_cpu:

@@ -18,6 +18,8 @@
 //     segment writes a partial, and a second pass adds each row's partials in
 //     segment order (several waves per row, deterministic y).
 
+#include <algorithm>
+
 #include "format.hpp"
 #include "kernels.hpp"
 #include "scan.hpp"
@@ -209,6 +211,37 @@ __global__ void __launch_bounds__(kBlock) csr_binned_kernel(const int32_t *__res
 	csr_block<BNNZ_MAX>(rp, col, val, x, y, blocks[w], blocks[w + 1], split_nnz, true, lanes_cfg, prod[wv]);
 }
 
+// Persistent variant: a grid of resident workgroups whose waves take blocks
+// w, w + W, w + 2W, ... (W waves in all, so at any time the waves work on W
+// consecutive blocks), with block w + W's stream requested while block w is
+// reduced (kernels.hpp: csr_stage_*).
+template <int BNNZ_MAX>
+__global__ void __launch_bounds__(kBlock) csr_binned_pipe_kernel(const int32_t *__restrict__ rp,
+		const int32_t *__restrict__ col, const double *__restrict__ val, const double *__restrict__ x,
+		double *__restrict__ y, const int32_t *__restrict__ blocks, long nblocks, int split_nnz, int lanes_cfg)
+{
+	__shared__ double prod[kWavesPerBlock][2 * BNNZ_MAX];
+	const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+	const long W = (long) gridDim.x * kWavesPerBlock;
+	long w = (long) blockIdx.x * kWavesPerBlock + wv;
+	if (w >= nblocks)
+		return; // waves never synchronise with each other: early exit is safe
+	CsrStage cur, nxt;
+	csr_stage_load<BNNZ_MAX>(cur, rp, col, val, blocks[w], blocks[w + 1]);
+	for (;;) {
+		double xv[4];
+		csr_stage_gather(cur, x, xv);
+		const long wn = w + W;
+		if (wn < nblocks)
+			csr_stage_load<BNNZ_MAX>(nxt, rp, col, val, blocks[wn], blocks[wn + 1]);
+		csr_stage_finish<BNNZ_MAX>(cur, xv, col, val, x, y, split_nnz, true, lanes_cfg, prod[wv]);
+		if (wn >= nblocks)
+			break;
+		cur = nxt;
+		w = wn;
+	}
+}
+
 __global__ void __launch_bounds__(kBlock) csr_split_kernel(const int32_t *__restrict__ col, const double *__restrict__ val,
 		const double *__restrict__ x, const int32_t *__restrict__ desc, long nsegs, double *__restrict__ partials)
 {
@@ -244,17 +277,42 @@ int csr_vec_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 	}
 	const int remap = xcd_chunk(f->p.xcd_remap);
 	const unsigned grid = xcd_padded_grid(grid_for(f->nblocks, kWavesPerBlock), remap);
+	if (f->p.pipeline) {
+		// persistent grid: the resident workgroups of the kernel instance
+		if (f->pipe_grid == 0) {
+			int dev = 0, cus = 0, per_cu = 0;
+			SPMVHIP_CHECK(hipGetDevice(&dev));
+			SPMVHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+			const void *k = f->block_nnz <= 256 ? (const void *) csr_binned_pipe_kernel<256>
+					: f->block_nnz <= 512 ? (const void *) csr_binned_pipe_kernel<512>
+					: (const void *) csr_binned_pipe_kernel<1024>;
+			SPMVHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, 0));
+			f->pipe_grid = (unsigned) std::max(1, cus * std::max(per_cu, 1));
+		}
+		const unsigned pgrid = std::min<unsigned>(grid_for(f->nblocks, kWavesPerBlock), f->pipe_grid);
+#define LAUNCH_PIPE(B)                                                                                          \
+	csr_binned_pipe_kernel<B><<<pgrid, kBlock, 0, s>>>(f->row_ptr.as<int32_t>(), f->col.as<int32_t>(),       \
+			f->val.as<double>(), x, y, f->blocks.as<int32_t>(), f->nblocks, f->split_nnz, f->lanes)
+		if (f->block_nnz <= 256)
+			LAUNCH_PIPE(256);
+		else if (f->block_nnz <= 512)
+			LAUNCH_PIPE(512);
+		else
+			LAUNCH_PIPE(1024);
+#undef LAUNCH_PIPE
+	} else {
 #define LAUNCH_BINNED(B)                                                                                    \
 	csr_binned_kernel<B><<<grid, kBlock, 0, s>>>(f->row_ptr.as<int32_t>(), f->col.as<int32_t>(),        \
 			f->val.as<double>(), x, y, f->blocks.as<int32_t>(), f->nblocks, f->split_nnz, f->lanes, \
 			remap)
-	if (f->block_nnz <= 256)
-		LAUNCH_BINNED(256);
-	else if (f->block_nnz <= 512)
-		LAUNCH_BINNED(512);
-	else
-		LAUNCH_BINNED(1024);
+		if (f->block_nnz <= 256)
+			LAUNCH_BINNED(256);
+		else if (f->block_nnz <= 512)
+			LAUNCH_BINNED(512);
+		else
+			LAUNCH_BINNED(1024);
 #undef LAUNCH_BINNED
+	}
 	SPMVHIP_CHECK(hipGetLastError());
 	if (f->nsegs > 0) {
 		csr_split_finalize<<<grid_for(f->nsplit, kBlock), kBlock, 0, s>>>(f->split_rows.as<int32_t>(),

@@ -21,16 +21,25 @@
 // coordinates of that value (element-interleaved, so the lanes read them
 // coalesced) accumulates value * x[col] into an LDS copy of the packet's
 // <= 255 rows. At the end of the packet the LDS rows are added to y.
+//
+// Three kernels implement this: div_spmv_lds_kernel (the packet copied to
+// LDS with coalesced 16-byte loads, the decode reading LDS only; used when
+// the largest packet fits), div_spmv_kernel (the same decode reading the
+// stream from global memory, for larger packets) and div_export_kernel, the
+// byte-at-a-time decode that exports the decoded (row, col,
+// value) sequence for the parity checks.
 
 #include "format.hpp"
 #include "../div/div_codec.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
 namespace spmvhip {
 
 constexpr int kDivWavesPerBlock = kWavesPerBlock;
+constexpr long kDivLdsMax = 65536; // LDS of the staged decoder without raising the per-kernel limit
 
 __device__ __forceinline__ uint64_t load_le(const uint8_t *p, int nbytes)
 {
@@ -46,28 +55,22 @@ __device__ __forceinline__ void wave_fence()
 	__builtin_amdgcn_wave_barrier();
 }
 
-// MODE 0: y += decoded products (spmv). MODE 1: write the decoded
-// coordinates and values in packet order (rows/cols/vals at pk_nnz + index).
-template <int MODE>
-__global__ void __launch_bounds__(kBlock) div_kernel(const uint8_t *__restrict__ bytes, const int64_t *__restrict__ pk_off,
-		const int64_t *__restrict__ pk_nnz, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
-		long ngroups, int L, const double *__restrict__ x, double *__restrict__ y, int32_t *__restrict__ rows_out,
-		int32_t *__restrict__ cols_out, double *__restrict__ vals_out)
+// Export of the decoded (row, col, value) sequence in packet order
+// (rows/cols/vals at pk_nnz + index): the decode stated byte by byte, for the
+// parity checks of the device decoder.
+__global__ void __launch_bounds__(kBlock) div_export_kernel(const uint8_t *__restrict__ bytes,
+		const int64_t *__restrict__ pk_off, const int64_t *__restrict__ pk_nnz, const int32_t *__restrict__ grp_pk,
+		long ngroups, int L, int32_t *__restrict__ rows_out, int32_t *__restrict__ cols_out,
+		double *__restrict__ vals_out)
 {
-	__shared__ double yl[kDivWavesPerBlock][256];
 	const int lane = lane_id(), wv = threadIdx.x / kWave;
 	const long g = (long) blockIdx.x * kDivWavesPerBlock + wv;
 	if (g >= ngroups)
 		return; // waves do not synchronise with each other
-	if (MODE == 0) {
-		for (long r = grp_row[g] + lane; r < grp_row[g + 1]; r += kWave)
-			y[r] = 0.0;
-		wave_fence();
-	}
 	for (int p = grp_pk[g]; p < grp_pk[g + 1]; p++) {
 		const uint8_t *pk = bytes + pk_off[p];
 		const uint32_t *h = (const uint32_t *) pk;
-		const uint32_t nv = h[0], nu = h[1], nrows = h[2], row_min = h[3], col_min = h[4];
+		const uint32_t nv = h[0], nu = h[1], row_min = h[3], col_min = h[4];
 		const int row_bits = pk[20], col_bits = pk[21];
 		const uint32_t nrfs = h[6];
 		const int cb = (row_bits + col_bits + 7) >> 3;
@@ -86,13 +89,7 @@ __global__ void __launch_bounds__(kBlock) div_kernel(const uint8_t *__restrict__
 		const uint8_t *coords = pk + 28 + 4 * L + 8 * (long) nrfs;
 		const uint8_t *lens = coords + (long) nv * cb;
 		const uint8_t *stream = lens + nu + lane_base;
-		const long nnz0 = MODE == 1 ? pk_nnz[p] : 0;
-
-		if (MODE == 0) {
-			for (uint32_t r = lane; r < nrows; r += kWave)
-				yl[wv][r] = 0.0;
-			wave_fence();
-		}
+		const long nnz0 = pk_nnz[p];
 		long cursor = 0, ks = 0, cs = 0;
 		for (uint32_t gi = 0; gi < nrfs; gi++) {
 			const long rf = rfs[gi], n = per_rf[gi];
@@ -112,15 +109,136 @@ __global__ void __launch_bounds__(kBlock) div_kernel(const uint8_t *__restrict__
 					for (long t = 0; t < rf; t++) {
 						const long cpos = pp < dv ? (pp * rf + t) * L + lane : mul_c + lane * rf + t;
 						const uint64_t c = load_le(coords + (cs + cpos) * cb, cb);
-						const uint32_t rr = (uint32_t) (c & row_mask);
-						const uint32_t cc = (uint32_t) (c >> row_bits);
-						if (MODE == 0) {
-							atomicAdd(&yl[wv][rr], d * x[col_min + cc]);
-						} else {
-							const long o = nnz0 + cs + cpos;
-							rows_out[o] = (int32_t) (row_min + rr);
-							cols_out[o] = (int32_t) (col_min + cc);
-							vals_out[o] = d;
+						const long o = nnz0 + cs + cpos;
+						rows_out[o] = (int32_t) (row_min + (uint32_t) (c & row_mask));
+						cols_out[o] = (int32_t) (col_min + (uint32_t) (c >> row_bits));
+						vals_out[o] = d;
+					}
+				}
+			}
+			ks += n;
+			cs += n * rf;
+		}
+	}
+}
+
+// nbytes (0..8) little-endian bytes at byte offset pos of an 8-byte aligned
+// base: two aligned 8-byte loads and a funnel shift instead of a byte loop.
+// The device stream carries 16 bytes of padding, so base + pos + 16 stays in
+// the buffer.
+__device__ __forceinline__ uint64_t load_le_fast(const uint8_t *__restrict__ base, long pos, int nbytes)
+{
+	const uint64_t *w = (const uint64_t *) (base + (pos & ~7L));
+	const int sh = (int) (pos & 7) * 8;
+	const uint64_t w0 = w[0], w1 = w[1];
+	const uint64_t v = sh ? (w0 >> sh) | (w1 << (64 - sh)) : w0;
+	return nbytes >= 8 ? v : v & ((1ull << (8 * nbytes)) - 1);
+}
+
+// y = A x from the packet stream, the decode restructured for latency: each
+// lane takes its values in chunks of K; the K length bytes are loaded
+// together (coalesced across lanes), the K stream offsets follow from a
+// running sum of the lengths in registers, then the K deltas and their
+// coordinates are requested together and the value prefix sum runs in
+// registers. Products go to the packet's rows in LDS (atomic adds; the
+// order of the additions is not fixed), which are added to y at the end of
+// the packet.
+template <int K>
+__global__ void __launch_bounds__(kBlock) div_spmv_kernel(const uint8_t *__restrict__ bytes,
+		const int64_t *__restrict__ pk_off, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
+		long ngroups, int L, const double *__restrict__ x, double *__restrict__ y)
+{
+	__shared__ double yl[kDivWavesPerBlock][256];
+	const int lane = lane_id(), wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+	const long g = (long) blockIdx.x * kDivWavesPerBlock + wv;
+	if (g >= ngroups)
+		return; // waves do not synchronise with each other
+	for (long r = grp_row[g] + lane; r < grp_row[g + 1]; r += kWave)
+		y[r] = 0.0;
+	wave_fence();
+	double *yw = yl[wv];
+	for (int p = grp_pk[g]; p < grp_pk[g + 1]; p++) {
+		const uint8_t *pk = bytes + pk_off[p];
+		const uint32_t *h = (const uint32_t *) pk;
+		const uint32_t nv = h[0], nu = h[1], nrows = h[2], row_min = h[3], col_min = h[4];
+		const int row_bits = pk[20], col_bits = pk[21];
+		const uint32_t nrfs = h[6];
+		const int cb = (row_bits + col_bits + 7) >> 3;
+		const uint64_t row_mask = row_bits ? ((1ull << row_bits) - 1) : 0;
+		long my_size = lane < L ? (long) h[7 + lane] : 0;
+		long incl = my_size;
+		for (int off = 1; off < kWave; off <<= 1) {
+			long o = __shfl_up(incl, off, kWave);
+			if (lane >= off)
+				incl += o;
+		}
+		const uint32_t *rfs = h + 7 + L;
+		const uint32_t *per_rf = rfs + nrfs;
+		const long coords_at = 28 + 4 * (long) L + 8 * (long) nrfs; // byte offsets in the packet
+		const long lens_at = coords_at + (long) nv * cb;
+		const long stream_at = lens_at + nu + (incl - my_size); // this lane's value stream
+		for (uint32_t r = lane; r < nrows; r += kWave)
+			yw[r] = 0.0;
+		wave_fence();
+		long cursor = 0, ks = 0, cs = 0;
+		for (uint32_t gi = 0; gi < nrfs; gi++) {
+			const long rf = rfs[gi], n = per_rf[gi];
+			const long dv = n / L, md = n % L, mul_c = dv * rf * L;
+			const long cnt = lane < L ? dv + (lane < md ? 1 : 0) : 0;
+			const long steps = dv + (md ? 1 : 0);
+			uint64_t val = 0;
+			for (long p0 = 0; p0 < steps; p0 += K) {
+				int len[K], tz[K];
+				long off[K];
+#pragma unroll
+				for (int k = 0; k < K; k++) {
+					const long pp = p0 + k;
+					const uint8_t lb = pp < cnt ? pk[lens_at + ks + pp * L + lane] : 0;
+					len[k] = lb & 15;
+					tz[k] = (lb >> 4) << 2;
+				}
+#pragma unroll
+				for (int k = 0; k < K; k++) {
+					off[k] = cursor;
+					cursor += len[k];
+				}
+				uint64_t dlt[K];
+#pragma unroll
+				for (int k = 0; k < K; k++)
+					dlt[k] = len[k] ? load_le_fast(pk, stream_at + off[k], len[k]) << tz[k] : 0;
+				if (rf == 1) {
+					uint64_t c[K];
+#pragma unroll
+					for (int k = 0; k < K; k++) {
+						const long pp = p0 + k;
+						c[k] = pp < cnt ? load_le_fast(pk, coords_at + (cs + pp * L + lane) * cb, cb) : 0;
+					}
+					double xv[K];
+#pragma unroll
+					for (int k = 0; k < K; k++)
+						xv[k] = x[col_min + (uint32_t) (c[k] >> row_bits)];
+#pragma unroll
+					for (int k = 0; k < K; k++) {
+						val += dlt[k];
+						if (p0 + k < cnt) {
+							double d;
+							__builtin_memcpy(&d, &val, 8);
+							atomicAdd(&yw[(uint32_t) (c[k] & row_mask)], d * xv[k]);
+						}
+					}
+				} else {
+#pragma unroll
+					for (int k = 0; k < K; k++) {
+						const long pp = p0 + k;
+						val += dlt[k];
+						if (pp >= cnt)
+							continue;
+						double d;
+						__builtin_memcpy(&d, &val, 8);
+						for (long t = 0; t < rf; t++) {
+							const long cpos = pp < dv ? (pp * rf + t) * L + lane : mul_c + lane * rf + t;
+							const uint64_t c = load_le_fast(pk, coords_at + (cs + cpos) * cb, cb);
+							atomicAdd(&yw[(uint32_t) (c & row_mask)], d * x[col_min + (uint32_t) (c >> row_bits)]);
 						}
 					}
 				}
@@ -128,12 +246,144 @@ __global__ void __launch_bounds__(kBlock) div_kernel(const uint8_t *__restrict__
 			ks += n;
 			cs += n * rf;
 		}
-		if (MODE == 0) {
-			wave_fence();
-			for (uint32_t r = lane; r < nrows; r += kWave)
-				y[row_min + r] += yl[wv][r];
-			wave_fence();
+		wave_fence();
+		for (uint32_t r = lane; r < nrows; r += kWave)
+			y[row_min + r] += yw[r];
+		wave_fence();
+	}
+}
+
+// The decoder with every packet staged in LDS (one wavefront per workgroup,
+// dynamic LDS = the largest packet rounded up + 16 bytes of padding, plus
+// the packet's 256 row accumulators). A packet is copied to LDS with
+// coalesced 16-byte loads, all requested before the first LDS write; the
+// other resident waves hide the latency. All stream, length and coordinate
+// reads of the decode are LDS reads; only the x gathers go to global memory.
+// The arithmetic is div_spmv_kernel's.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kDivStageRegs = 4; // 16-byte loads per lane in flight per round of the packet copy
+
+__device__ __forceinline__ uint64_t lds_le(const uint8_t *base, long pos, int nbytes)
+{
+	const uint64_t *w = (const uint64_t *) (base + (pos & ~7L));
+	const int sh = (int) (pos & 7) * 8;
+	const uint64_t w0 = w[0], w1 = w[1];
+	const uint64_t v = sh ? (w0 >> sh) | (w1 << (64 - sh)) : w0;
+	return nbytes >= 8 ? v : v & ((1ull << (8 * nbytes)) - 1);
+}
+
+template <int K>
+__global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__restrict__ bytes,
+		const int64_t *__restrict__ pk_off, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
+		long ngroups, int L, int buf_bytes, const double *__restrict__ x, double *__restrict__ y)
+{
+	extern __shared__ u32x4 smem[];
+	uint8_t *pk = (uint8_t *) smem;                   // buf_bytes (a multiple of 16, padding included)
+	double *yw = (double *) (pk + buf_bytes);          // 256 row accumulators
+	const int lane = lane_id();
+	const long g = blockIdx.x;
+	if (g >= ngroups)
+		return;
+	for (long r = grp_row[g] + lane; r < grp_row[g + 1]; r += kWave)
+		y[r] = 0.0;
+	const int p_end = grp_pk[g + 1];
+	int p = grp_pk[g];
+	for (; p < p_end; p++) {
+		// copy the packet to LDS in rounds of kDivStageRegs coalesced 16-byte
+		// loads per lane, all of a round in flight before its LDS writes
+		const u32x4 *src = (const u32x4 *) (bytes + pk_off[p]);
+		const int n16 = (int) ((pk_off[p + 1] - pk_off[p] + 15) >> 4);
+		for (int j0 = 0; j0 < n16; j0 += kDivStageRegs * kWave) {
+			u32x4 stage[kDivStageRegs];
+#pragma unroll
+			for (int i = 0; i < kDivStageRegs; i++) {
+				const int j = j0 + i * kWave + lane;
+				if (j < n16)
+					stage[i] = __builtin_nontemporal_load(src + j);
+			}
+#pragma unroll
+			for (int i = 0; i < kDivStageRegs; i++) {
+				const int j = j0 + i * kWave + lane;
+				if (j < n16)
+					smem[j] = stage[i];
+			}
 		}
+		if (lane == 0) // zero padding after the packet: the decode reads whole words
+			smem[n16] = u32x4{0, 0, 0, 0};
+		wave_fence();
+		const uint32_t *h = (const uint32_t *) pk;
+		const uint32_t nv = h[0], nu = h[1], nrows = h[2], row_min = h[3], col_min = h[4];
+		const int row_bits = pk[20], col_bits = pk[21];
+		const uint32_t nrfs = h[6];
+		const int cb = (row_bits + col_bits + 7) >> 3;
+		const uint64_t row_mask = row_bits ? ((1ull << row_bits) - 1) : 0;
+		long my_size = lane < L ? (long) h[7 + lane] : 0;
+		long incl = my_size;
+		for (int off = 1; off < kWave; off <<= 1) {
+			long o = __shfl_up(incl, off, kWave);
+			if (lane >= off)
+				incl += o;
+		}
+		const uint32_t *rfs = h + 7 + L;
+		const uint32_t *per_rf = rfs + nrfs;
+		const long coords_at = 28 + 4 * (long) L + 8 * (long) nrfs;
+		const long lens_at = coords_at + (long) nv * cb;
+		const long stream_at = lens_at + nu + (incl - my_size);
+		for (uint32_t r = lane; r < nrows; r += kWave)
+			yw[r] = 0.0;
+		wave_fence();
+		long cursor = 0, ks = 0, cs = 0;
+		for (uint32_t gi = 0; gi < nrfs; gi++) {
+			const long rf = rfs[gi], n = per_rf[gi];
+			const long dv = n / L, md = n % L, mul_c = dv * rf * L;
+			const long cnt = lane < L ? dv + (lane < md ? 1 : 0) : 0;
+			const long steps = dv + (md ? 1 : 0);
+			uint64_t val = 0;
+			for (long p0 = 0; p0 < steps; p0 += K) {
+				uint64_t c[K];
+				double xv[K];
+				if (rf == 1) {
+#pragma unroll
+					for (int k = 0; k < K; k++) {
+						const long pp = p0 + k;
+						c[k] = pp < cnt ? lds_le(pk, coords_at + (cs + pp * L + lane) * cb, cb) : 0;
+					}
+#pragma unroll
+					for (int k = 0; k < K; k++)
+						xv[k] = x[col_min + (uint32_t) (c[k] >> row_bits)];
+				}
+#pragma unroll
+				for (int k = 0; k < K; k++) {
+					const long pp = p0 + k;
+					if (pp >= cnt)
+						continue;
+					const uint8_t lb = pk[lens_at + ks + pp * L + lane];
+					const int len = lb & 15, tz = (lb >> 4) << 2;
+					if (len) {
+						val += lds_le(pk, stream_at + cursor, len) << tz;
+						cursor += len;
+					}
+					double d;
+					__builtin_memcpy(&d, &val, 8);
+					if (rf == 1) {
+						atomicAdd(&yw[(uint32_t) (c[k] & row_mask)], d * xv[k]);
+					} else {
+						for (long t = 0; t < rf; t++) {
+							const long cpos = pp < dv ? (pp * rf + t) * L + lane : mul_c + lane * rf + t;
+							const uint64_t cc = lds_le(pk, coords_at + (cs + cpos) * cb, cb);
+							atomicAdd(&yw[(uint32_t) (cc & row_mask)],
+									d * x[col_min + (uint32_t) (cc >> row_bits)]);
+						}
+					}
+				}
+			}
+			ks += n;
+			cs += n * rf;
+		}
+		wave_fence();
+		for (uint32_t r = lane; r < nrows; r += kWave)
+			y[row_min + r] += yw[r];
+		wave_fence(); // the packet buffer and accumulators are reused
 	}
 }
 
@@ -172,12 +422,18 @@ int div_convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col, co
 		at += (st.pk_off[p + 1] - st.pk_off[p] + 15) / 16 * 16;
 	}
 	doff[np] = at;
-	std::vector<uint8_t> dev(at > 0 ? at : 16, 0);
+	std::vector<uint8_t> dev(at + 16, 0); // +16: the decoder reads whole aligned 8-byte words
 	for (long p = 0; p < np; p++)
 		memcpy(&dev[doff[p]], &st.bytes[st.pk_off[p]], st.pk_off[p + 1] - st.pk_off[p]);
 	f->div_npackets = np;
 	f->div_ngroups = ng;
 	f->div_bytes = (long) st.bytes.size();
+	int64_t max_pk = 0;
+	for (long p = 0; p < np; p++)
+		max_pk = std::max<int64_t>(max_pk, doff[p + 1] - doff[p]);
+	// the staged decoder keeps a packet (16-byte rounded, + 16 of zero padding)
+	// and 256 row accumulators in LDS: used while that fits 64 KiB
+	f->div_buf_bytes = max_pk + 16 + 2048 <= kDivLdsMax ? (int) (max_pk + 16) : 0;
 	SPMVHIP_TRY(f->div_stream.alloc(dev.size()));
 	SPMVHIP_TRY(f->div_pk_off.alloc((np + 1) * sizeof(int64_t)));
 	SPMVHIP_TRY(f->div_pk_nnz.alloc((np + 1) * sizeof(int64_t)));
@@ -207,9 +463,21 @@ int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 		SPMVHIP_CHECK(hipGetLastError());
 		return SPMVHIP_OK;
 	}
-	div_kernel<0><<<grid_for(f->div_ngroups, kDivWavesPerBlock), kBlock, 0, s>>>(f->div_stream.as<uint8_t>(),
-			f->div_pk_off.as<int64_t>(), f->div_pk_nnz.as<int64_t>(), f->div_groups.as<int32_t>(),
-			f->div_group_rows.as<int32_t>(), f->div_ngroups, f->div_lanes, x, y, nullptr, nullptr, nullptr);
+	if (f->div_buf_bytes > 0) {
+		const size_t lds = (size_t) f->div_buf_bytes + 256 * sizeof(double);
+		if (f->p.unroll >= 8)
+			div_spmv_lds_kernel<8><<<(unsigned) f->div_ngroups, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
+					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
+					f->div_ngroups, f->div_lanes, f->div_buf_bytes, x, y);
+		else
+			div_spmv_lds_kernel<4><<<(unsigned) f->div_ngroups, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
+					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
+					f->div_ngroups, f->div_lanes, f->div_buf_bytes, x, y);
+	} else {
+		div_spmv_kernel<8><<<grid_for(f->div_ngroups, kDivWavesPerBlock), kBlock, 0, s>>>(f->div_stream.as<uint8_t>(),
+				f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
+				f->div_ngroups, f->div_lanes, x, y);
+	}
 	SPMVHIP_CHECK(hipGetLastError());
 	return SPMVHIP_OK;
 }
@@ -231,10 +499,10 @@ int div_export_decoded(const spmvhip_format *f, int which, void *buf, size_t *by
 	SPMVHIP_TRY(rows.alloc(f->nnz * sizeof(int32_t)));
 	SPMVHIP_TRY(cols.alloc(f->nnz * sizeof(int32_t)));
 	SPMVHIP_TRY(vals.alloc(f->nnz * sizeof(double)));
-	div_kernel<1><<<grid_for(f->div_ngroups, kDivWavesPerBlock), kBlock, 0, f->stream>>>(f->div_stream.as<uint8_t>(),
-			f->div_pk_off.as<int64_t>(), f->div_pk_nnz.as<int64_t>(), f->div_groups.as<int32_t>(),
-			f->div_group_rows.as<int32_t>(), f->div_ngroups, f->div_lanes, nullptr, nullptr, rows.as<int32_t>(),
-			cols.as<int32_t>(), vals.as<double>());
+	div_export_kernel<<<grid_for(f->div_ngroups, kDivWavesPerBlock), kBlock, 0, f->stream>>>(
+			f->div_stream.as<uint8_t>(), f->div_pk_off.as<int64_t>(), f->div_pk_nnz.as<int64_t>(),
+			f->div_groups.as<int32_t>(), f->div_ngroups, f->div_lanes, rows.as<int32_t>(), cols.as<int32_t>(),
+			vals.as<double>());
 	SPMVHIP_CHECK(hipGetLastError());
 	SPMVHIP_CHECK(hipStreamSynchronize(f->stream));
 	const void *src = which == 6 ? rows.p : which == 7 ? cols.p : vals.p;

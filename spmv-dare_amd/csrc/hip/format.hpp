@@ -18,6 +18,7 @@ struct spmvhip_format {
 	spmvhip::DevBuf blocks;
 	long nblocks = 0;
 	int block_nnz = 512, split_nnz = 8192, lanes = 0;
+	unsigned pipe_grid = 0;       // resident workgroups of the persistent CSR kernel (set on first use)
 	spmvhip::DevBuf seg_desc;     // int32 triplets (row, j_begin, j_end) per segment
 	spmvhip::DevBuf seg_ptr;      // int32, nsplit + 1
 	spmvhip::DevBuf split_rows;   // int32, nsplit
@@ -45,6 +46,7 @@ struct spmvhip_format {
 	spmvhip::DevBuf div_group_rows; // int32 per packet group: [row_begin,row_end)
 	long div_npackets = 0, div_ngroups = 0, div_bytes = 0;
 	int div_lanes = 64;
+	int div_buf_bytes = 0;        // LDS packet buffer of the staged decoder: largest packet + 16, or 0 if too large
 
 	// host-pointer contract
 	const double *bound_x = nullptr;

@@ -141,6 +141,114 @@ __device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const 
 	wave_lds_sync(); // the buffer is reused by the wave's next block
 }
 
+// The same row block split in three stages, so that a persistent wave can
+// request block i + 1's indices and values (csr_stage_load) after block i's
+// gathers (csr_stage_gather) and before block i's products, LDS reduction
+// and stores (csr_stage_finish). Loads complete in issue order, so this
+// order lets the wave wait for block i's gathers while block i + 1's stream
+// is still in flight. The arithmetic is csr_block's, entry for entry.
+struct CsrStage {
+	int r0, nrows, j0, nz, ra;
+	bool long_row; // one row longer than BNNZ_MAX: summed by wave_row_sum
+	double v[4];
+	int32_t c[4];
+};
+
+template <int BNNZ_MAX>
+__device__ __forceinline__ void csr_stage_load(CsrStage &st, const int32_t *__restrict__ rp,
+		const int32_t *__restrict__ col, const double *__restrict__ val, int r0, int r1)
+{
+	const int lane = lane_id();
+	st.r0 = __builtin_amdgcn_readfirstlane(r0);
+	r1 = __builtin_amdgcn_readfirstlane(r1);
+	st.j0 = __builtin_amdgcn_readfirstlane(rp[st.r0]);
+	st.nz = __builtin_amdgcn_readfirstlane(rp[r1]) - st.j0;
+	st.nrows = r1 - st.r0;
+	st.long_row = st.nrows == 1 && st.nz > BNNZ_MAX;
+	if (st.long_row)
+		return;
+	st.ra = rp[st.r0 + (lane < st.nrows ? lane : st.nrows)] - st.j0;
+#pragma unroll
+	for (int u = 0; u < 4; u++) {
+		const int e = u * kWave + lane;
+		const int ec = e < st.nz ? e : st.nz - 1;
+		st.v[u] = __builtin_nontemporal_load(val + st.j0 + ec);
+		st.c[u] = __builtin_nontemporal_load(col + st.j0 + ec);
+	}
+}
+
+__device__ __forceinline__ void csr_stage_gather(const CsrStage &st, const double *__restrict__ x, double xv[4])
+{
+	if (st.long_row)
+		return;
+#pragma unroll
+	for (int u = 0; u < 4; u++)
+		xv[u] = x[st.c[u]];
+}
+
+template <int BNNZ_MAX>
+__device__ __forceinline__ void csr_stage_finish(const CsrStage &st, const double xv[4],
+		const int32_t *__restrict__ col, const double *__restrict__ val, const double *__restrict__ x,
+		double *__restrict__ y, int split_nnz, bool skip_split, int lanes_cfg, double *p)
+{
+	const int lane = lane_id();
+	const int nz = st.nz, nrows = st.nrows, j0 = st.j0, r0 = st.r0;
+	if (st.long_row) {
+		if (skip_split && nz > split_nnz)
+			return;
+		double sum = wave_row_sum(col, val, x, j0, j0 + nz);
+		if (lane == 0)
+			y[r0] = sum;
+		return;
+	}
+#pragma unroll
+	for (int u = 0; u < 4; u++) {
+		const int e = u * kWave + lane;
+		p[e] = e < nz ? st.v[u] * xv[u] : 0.0;
+	}
+	// entries past the first 256 (blocks hold fewer than 2 * BNNZ_MAX)
+	for (int e0 = 4 * kWave; e0 < nz; e0 += 4 * kWave) {
+		double v[4], xw[4];
+		int32_t c[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int e = e0 + u * kWave + lane;
+			const int ec = e < nz ? e : nz - 1;
+			v[u] = __builtin_nontemporal_load(val + j0 + ec);
+			c[u] = __builtin_nontemporal_load(col + j0 + ec);
+		}
+#pragma unroll
+		for (int u = 0; u < 4; u++)
+			xw[u] = x[c[u]];
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int e = e0 + u * kWave + lane;
+			p[e] = e < nz ? v[u] * xw[u] : 0.0;
+		}
+	}
+	wave_lds_sync();
+	int lg = 6;
+	while (lg > 0 && (nrows << lg) > kWave)
+		lg--;
+	if (lanes_cfg > 0 && lanes_cfg < (1 << lg))
+		lg = 31 - __builtin_clz(lanes_cfg);
+	const int L = 1 << lg, g = lane >> lg, sub = lane & (L - 1), gshift = 6 - lg;
+	const int rows_up = (nrows + (1 << gshift) - 1) >> gshift << gshift;
+	for (int r = g; r < rows_up; r += 1 << gshift) {
+		const int a = __shfl(st.ra, r & (kWave - 1), kWave);
+		const int bn = __shfl(st.ra, (r + 1) & (kWave - 1), kWave);
+		const int b = r + 1 < kWave ? bn : nz;
+		double s = 0;
+		if (r < nrows)
+			for (int e = a + sub; e < b; e += L)
+				s += p[e];
+		s = group_sum(s, L);
+		if (r < nrows && sub == 0)
+			y[r0 + r] = s;
+	}
+	wave_lds_sync();
+}
+
 // x[c] for a SELL entry. With XW the workgroup has copied x[lo, lo + W) to
 // xs (LDS): columns in that window are read from LDS, any other column (a
 // padding entry's, or all of them when W = 0) from x.

@@ -35,7 +35,7 @@ def evaluate(sel, Xte, Tte, Ttr, k=3):
     topk = order[:, :k]
     refined = topk[np.arange(len(topk)), np.argmin(Tte[np.arange(len(topk))[:, None], topk], axis=1)]
     static = int(np.argmin(np.log(Ttr / Ttr.min(axis=1, keepdims=True)).mean(axis=0)))
-    default = tuning.VARIANTS.index(("csr_vec", {"block_nnz": 512, "lanes_per_row": 0}))
+    default = tuning.VARIANTS.index(("csr_vec", {"block_nnz": 256, "lanes_per_row": 0}))
     return {
         "mlp_top1": summary(regrets(Tte, top1)),
         "mlp_top%d_refined" % k: summary(regrets(Tte, refined)),

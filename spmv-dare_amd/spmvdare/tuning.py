@@ -28,7 +28,7 @@ def _variants():
         for lanes in (0, 1):
             v.append(("csr_vec", {"block_nnz": bnnz, "lanes_per_row": lanes}))
     for bnnz in (128, 256):
-        v.append(("csr_vec", {"block_nnz": bnnz, "xcd_remap": 1}))
+        v.append(("csr_vec", {"block_nnz": bnnz, "lanes_per_row": 4}))
     # SELL-C-sigma, C = 64: sort window sigma x lanes per row T
     for sigma in (64, 256, 1024, 4096):
         for T in (1, 2, 4, 8):
@@ -37,13 +37,16 @@ def _variants():
         v.append(("sell_c_s", {"sigma": 16384, "lanes_per_row": T, "unroll": 4}))
     for unroll in (2, 8):
         v.append(("sell_c_s", {"sigma": 256, "lanes_per_row": 1, "unroll": unroll}))
-    for sigma in (32, 128, 512, 2048):
-        v.append(("sell_c_s", {"sigma": sigma, "lanes_per_row": 4, "unroll": 4}))
+    v.append(("sell_c_s", {"sigma": 4096, "lanes_per_row": 4, "unroll": 4, "y_mode": 3}))
+    # x windows in LDS (banded placements)
+    v.append(("sell_c_s", {"sigma": 256, "lanes_per_row": 4, "unroll": 4, "x_window": 1}))
+    v.append(("sell_c_s", {"sigma": 256, "lanes_per_row": 1, "unroll": 4, "x_window": 1}))
+    v.append(("sell_c_s", {"sigma": 64, "lanes_per_row": 4, "unroll": 4, "x_window": 4096}))
     # unsorted SELL
     for T in (1, 2, 4, 8):
         v.append(("sell", {"lanes_per_row": T, "unroll": 4}))
-    for unroll in (2, 8):
-        v.append(("sell", {"lanes_per_row": 1, "unroll": unroll}))
+    v.append(("sell", {"lanes_per_row": 1, "unroll": 8}))
+    v.append(("sell", {"lanes_per_row": 4, "unroll": 4, "x_window": 1}))
     return v
 
 
@@ -174,9 +177,13 @@ class Selector:
     def predict(self, X):
         return self.net.predict(self._norm(np.atleast_2d(X)).astype(np.float32))
 
-    def select(self, X, k=1):
-        """Indices of the k best predicted variants per row."""
-        return np.argsort(self.predict(X), axis=1)[:, :k]
+    def select(self, X, k=1, allowed=None):
+        """Indices of the k best predicted variants per row (only among the
+        variants where the boolean mask `allowed` is true, if given)."""
+        pred = self.predict(X)
+        if allowed is not None:
+            pred = np.where(np.asarray(allowed, dtype=bool)[None, :], pred, np.inf)
+        return np.argsort(pred, axis=1)[:, :k]
 
     def save(self, path):
         json.dump({"hidden": list(self.hidden), "mu": self.mu.tolist(), "sd": self.sd.tolist(),

@@ -117,7 +117,9 @@ def test_csr_naive_bit_exact(name):
 
 
 CSR_VEC_VARIANTS = [dict(), dict(lanes_per_row=1), dict(block_nnz=256, split_nnz=2048),
-                    dict(block_nnz=1024), dict(block_nnz=1024, split_nnz=65536), dict(xcd_remap=1)]
+                    dict(block_nnz=1024), dict(block_nnz=1024, split_nnz=65536), dict(xcd_remap=1),
+                    dict(pipeline=1), dict(pipeline=1, block_nnz=512, split_nnz=2048),
+                    dict(pipeline=1, block_nnz=128, lanes_per_row=4)]
 
 
 @pytest.mark.parametrize("name", ALL)

@@ -50,6 +50,8 @@ def test_trained_selector_beats_static_choice():
     if not (os.path.exists(tr_path) and os.path.exists(te_path)):
         pytest.skip("selector corpus not committed yet")
     tr, te = np.load(tr_path), np.load(te_path)
+    names = [tuning.variant_name(i) for i in range(len(tuning.VARIANTS))]
+    assert list(tr["variants"]) == names and list(te["variants"]) == names, "corpus of another variant set"
     sel = tuning.Selector(seed=1)
     losses = sel.fit(tr["features"], tr["times"], epochs=150, lr=2e-3)
     assert losses[-1] < losses[0]
