@@ -177,6 +177,126 @@ def probe(hip, np):
     print(json.dumps({"probe": out}), flush=True)
 
 
+E2E_RANGES = dict(rows=(1e4, 4e6), avg=(2, 128), skew=(1, 1000), distributions=("gamma",))
+
+
+def run_e2e(args, gen, hip, np):
+    """End-to-end streaming sample of the held-out workload (master seed 55:
+    rows = cols log-uniform in [1e4, 4e6], avg 2..128, gamma rows, skew up to
+    1000; the other descriptors as the batched workload). The first `count`
+    matrices of that stream are processed in chunks that fit HBM: the host
+    generates chunk i + 1 (a worker thread; the generator releases the
+    interpreter lock) while the GPU uploads chunk i, computes the structure
+    statistics, lets the selector choose each matrix's format, converts it and
+    runs the chunk as one batched launch. Compared on the same GPU with plain
+    per-matrix CSR launches; every y is checked against the device csr_naive
+    (bit-identical to the oracle's sequential sum) at 1e-13 normwise; the
+    first `e2e_tuned` matrices are also tuned exhaustively for the regret.
+    Prints one JSON line."""
+    import threading
+    from spmvdare import tuning
+    model = os.path.join(ROOT, "spmv-dare_amd", "spmvdare", "models", "selector.json")
+    sel = tuning.Selector.load(model)
+    allowed = np.array([not (k == "csr_vec" and kw.get("block_nnz", 256) > 256) for k, kw in tuning.VARIANTS])
+    csr = tuning.VARIANTS.index(("csr_vec", {"block_nnz": 256, "lanes_per_row": 0}))
+    count = args.e2e_count
+    params = tuning.sample_params(55, count * 2, **E2E_RANGES)
+    chunk = max(1, args.e2e_chunk)
+    stream = iter(params)
+
+    def produce(n):
+        out = []
+        for p in stream:
+            try:
+                out.append((p, gen.generate(p, stats=False)))
+            except gen.GenError:
+                continue
+            if len(out) == n:
+                break
+        return out
+
+    t_start = time.perf_counter()
+    done, totals = 0, {"nnz": 0, "bytes": 0, "ms_batched": 0.0, "ms_plain": 0.0, "gen_s": 0.0}
+    worst, regrets, chosen = 0.0, [], []
+    pending = produce(min(chunk, count))
+    while pending:
+        box = {}
+        want = min(chunk, count - done - len(pending))
+        worker = None
+        if want > 0:
+            worker = threading.Thread(target=lambda: box.update(m=produce(want)))
+            worker.start()
+        mats = []
+        for p, a in pending:
+            dm = tuning.DeviceCSR(a, p.seed + 1)
+            # the selector's top 3, refined by timing them (the fallback to tuning)
+            top = [int(i) for i in sel.select(tuning.features(p, dm.stats()), k=3, allowed=allowed)[0]]
+            tt = [tuning.time_variant(dm, i, 1, 3) for i in top]
+            v = top[int(np.argmin(tt))]
+            chosen.append(tuning.variant_name(v))
+            f_sel, f_csr = dm.convert(v), dm.convert(csr)
+            f_ref = hip.csr_to_format_device(dm.rp, dm.col, dm.val, dm.m, dm.n, dm.nnz, kind="csr_naive")
+            if done + len(mats) < args.e2e_tuned:
+                t = tuning.autotune(dm, warmup=1, iters=5)
+                regrets.append((float(tuning.time_variant(dm, top[0], 1, 5) / t.min() - 1.0),
+                                float(tuning.time_variant(dm, v, 1, 5) / t.min() - 1.0)))
+            mats.append((p, a, dm, f_sel, f_csr, f_ref, hip.DeviceArray(max(a.m, 1))))
+            totals["nnz"] += a.nnz
+            totals["bytes"] += alg_bytes(a.m, a.n, a.nnz)
+        batch, base = hip.Batch(args.tile), hip.Batch(args.tile)
+        batch.set_queues(args.queues)
+        for p, a, dm, f_sel, f_csr, f_ref, dy_csr in mats:
+            batch.add(f_sel, dm.dx, dm.dy)
+            base.add(f_csr, dm.dx, dy_csr)
+        batch.finalize(use_graph=True)
+        base.finalize(use_graph=False)
+        totals["ms_batched"] += float(np.median(batch.time(2, args.e2e_steps)))
+        totals["ms_plain"] += float(np.median(base.time(1, args.e2e_steps, per_matrix=True)))
+        for p, a, dm, f_sel, f_csr, f_ref, dy_csr in mats:
+            yb = dm.dy.download()[:a.m]
+            f_ref.spmv_device(dm.dx, dy_csr)
+            hip.synchronize()
+            yr = dy_csr.download()[:a.m]
+            nrm = np.linalg.norm(yr)
+            err = float(np.linalg.norm(yb - yr) / nrm) if nrm > 0 else float(np.linalg.norm(yb))
+            worst = max(worst, err)
+        done += len(mats)
+        batch.destroy()
+        base.destroy()
+        for m_ in mats:
+            for f in m_[3:6]:
+                f.destroy()
+        del mats
+        log("e2e: %d/%d matrices, %.0f s" % (done, count, time.perf_counter() - t_start))
+        if worker is not None:
+            worker.join()
+        pending = box.get("m", [])
+    t_b, t_p = totals["ms_batched"], totals["ms_plain"]
+    out = {
+        "metric": "end-to-end batched FP64 SpMV matrices/s (selector-chosen formats), streaming sample",
+        "value": round(done / (t_b * 1e-3), 1), "unit": "matrices/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic (reference generator restated bit-exactly)",
+        "config": {"workload": "first %d matrices of the master-seed-55 stream (rows 1e4..4e6, avg 2..128, gamma, "
+                               "skew <= 1000), chunks of %d" % (done, chunk),
+                   "nnz": totals["nnz"], "alg_bytes": totals["bytes"]},
+        "batched_ms": round(t_b, 3), "batched_gbps": round(totals["bytes"] / (t_b * 1e-3) / 1e9, 1),
+        "plain_per_matrix_csr": {"ms": round(t_p, 3), "matrices_per_s": round(done / (t_p * 1e-3), 1),
+                                 "gbps": round(totals["bytes"] / (t_p * 1e-3) / 1e9, 1)},
+        "speedup_vs_plain_csr": round(t_p / t_b, 3),
+        "max_normwise_err_vs_csr_naive": worst,
+        "selector_regret_vs_exhaustive": {
+            "matrices": len(regrets),
+            "top1_mean": float(np.mean([r[0] for r in regrets])) if regrets else None,
+            "top3_refined_mean": float(np.mean([r[1] for r in regrets])) if regrets else None,
+            "top3_refined_max": float(np.max([r[1] for r in regrets])) if regrets else None},
+        "wall_s": round(time.perf_counter() - t_start, 1),
+        "chosen_variants": {k: chosen.count(k) for k in sorted(set(chosen))},
+    }
+    log("e2e %d matrices: batched %.2f ms vs per-matrix CSR %.2f ms (x%.2f), max err %.2e" % (
+        done, t_b, t_p, t_p / t_b, worst))
+    print(json.dumps(out), flush=True)
+
+
 def run_batched(args, gen, hip, np):
     """Batched workload: `count` generated matrices of uneven sizes (rows =
     cols log-uniform in [1e3, 2e5], avg 4..64, ...; master seed 21), each with
@@ -222,6 +342,7 @@ def run_batched(args, gen, hip, np):
     log("batched: %d matrices, %d nnz, prepared in %.0f s; selector=%s" % (
         len(members), total_nnz, time.perf_counter() - t0, "mlp" if sel is not None else "fixed"))
     batch = hip.Batch(args.tile)
+    batch.set_queues(args.queues)
     base = hip.Batch(args.tile)
     for dm, f_sel, f_csr, dy_csr in members:
         batch.add(f_sel, dm.dx, dm.dy)
@@ -267,6 +388,13 @@ def main():
     ap.add_argument("--tile", type=int, default=0,
                     help="batched launch: units per work item (> 0), or about -tile nnz per item (< 0); 0 = 16384 nnz")
     ap.add_argument("--no-selector", action="store_true")
+    ap.add_argument("--queues", type=int, default=8, choices=(1, 8),
+                    help="batched launch: work queues (8 = one per XCD, 1 = shared)")
+    ap.add_argument("--e2e", action="store_true", help="end-to-end streaming sample of the held-out workload")
+    ap.add_argument("--e2e-count", type=int, default=32, help="matrices of the streaming sample")
+    ap.add_argument("--e2e-chunk", type=int, default=8, help="matrices per chunk of the streaming sample")
+    ap.add_argument("--e2e-tuned", type=int, default=4, help="matrices of the sample also tuned exhaustively")
+    ap.add_argument("--e2e-steps", type=int, default=10, help="timed launches per chunk")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=128)
     ap.add_argument("--warmup", type=int, default=10)
@@ -294,6 +422,9 @@ def main():
         return
     if args.batched:
         run_batched(args, gen, hip, np)
+        return
+    if args.e2e:
+        run_e2e(args, gen, hip, np)
         return
     line, desc = WORKLOADS[args.workload]
     p = gen.GenParams.parse(line)

@@ -186,6 +186,9 @@ void spmvhip_destroy(spmvhip_format *f);
  * nnz; < 0: items of about -tile_units nnz (at most 2^24). */
 typedef struct spmvhip_batch spmvhip_batch;
 int spmvhip_batch_create(int tile_units, spmvhip_batch **out);
+/* Work queues (before finalize): 8 (default) = one per XCD, matrices dealt to
+ * them by nnz; 1 = one shared queue in insertion order. */
+int spmvhip_batch_set_queues(spmvhip_batch *b, int nqueues);
 int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d_x, double *d_y);
 int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph);
 int spmvhip_batch_run(spmvhip_batch *b);          /* asynchronous on the batch's stream */

@@ -6,9 +6,12 @@
 //
 // Units are the formats' own per-wave work: a row block of the binned CSR
 // format or a 64-row slice of SELL / SELL-C-sigma (kernels.hpp). A work item
-// is `tile` consecutive units of one matrix, taken by one wavefront. Long CSR
-// rows are summed by one wave (the split path of the standalone kernel is
-// not used here), so any row length is handled in the single launch.
+// is consecutive units of one matrix (about 16k nonzeros, or `tile` units),
+// taken by one wavefront. CSR rows longer than the member's split_nnz are
+// cut into the standalone kernel's segments: segment work items write
+// partial sums, and batch_split_finalize (the next node of the same launch
+// sequence) adds each row's partials in segment order, so a heavy-tailed row
+// is spread over many waves instead of holding one.
 //
 // XCD-aware scheduling: the matrices are dealt to 8 queues, one per XCD,
 // balanced by nnz (largest first to the least loaded queue). Workgroup b is
@@ -39,10 +42,21 @@ struct BatchMat {
 	const double *sval;
 	const double *x;
 	double *y;
+	// CSR rows longer than split_nnz: segments (row, j0, j1) summed into
+	// partials by the persistent kernel, added per row by batch_split_finalize
+	const int32_t *seg_desc, *seg_ptr, *split_rows;
+	double *partials;
+	int split_nnz;
 };
 
+enum { kItemUnits = 0, kItemSegments = 1 };
+
 struct WorkItem {
-	int mat, u0, u1, pad;
+	int mat, u0, u1, type; // units (row blocks / slices) or split-row segments [u0, u1)
+};
+
+struct SplitRef {
+	int mat, i; // split row i of member mat
 };
 
 // zero the kXcds queue counters (a kernel, not a memset: the batch is
@@ -74,10 +88,16 @@ __global__ void __launch_bounds__(kBlock) batch_kernel(const BatchMat *__restric
 				break; // queue drained: wave-uniform exit, every wave reaches it
 			const WorkItem wi = items[q0 + t];
 			const BatchMat &M = mats[wi.mat];
-			if (M.kind == SPMVHIP_CSR_VEC) {
+			if (wi.type == kItemSegments) {
+				for (int u = wi.u0; u < wi.u1; u++) {
+					const double sum = wave_row_sum(M.col, M.val, M.x, M.seg_desc[3 * u + 1], M.seg_desc[3 * u + 2]);
+					if (lane_id() == 0)
+						M.partials[u] = sum;
+				}
+			} else if (M.kind == SPMVHIP_CSR_VEC) {
 				for (int u = wi.u0; u < wi.u1; u++)
-					csr_block<BNNZ>(M.rp, M.col, M.val, M.x, M.y, M.blocks[u], M.blocks[u + 1], 0, false,
-							M.lanes, prod[wv]);
+					csr_block<BNNZ>(M.rp, M.col, M.val, M.x, M.y, M.blocks[u], M.blocks[u + 1], M.split_nnz,
+							M.partials != nullptr, M.lanes, prod[wv]);
 			} else {
 				for (int u = wi.u0; u < wi.u1; u++)
 					sell_slice<4>(M.sptr, M.sw, M.scol, M.sval, M.order, M.x, M.y, M.m, u, M.lanes);
@@ -86,18 +106,35 @@ __global__ void __launch_bounds__(kBlock) batch_kernel(const BatchMat *__restric
 	}
 }
 
+// y[row] = sum of the row's segment partials in segment order (the
+// standalone csr_split_finalize, over every member of the batch)
+__global__ void batch_split_finalize(const BatchMat *__restrict__ mats, const SplitRef *__restrict__ refs, long n)
+{
+	const long k = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n)
+		return;
+	const BatchMat &M = mats[refs[k].mat];
+	const int i = refs[k].i;
+	double s = 0;
+	for (int j = M.seg_ptr[i]; j < M.seg_ptr[i + 1]; j++)
+		s += M.partials[j];
+	M.y[M.split_rows[i]] = s;
+}
+
 } // namespace spmvhip
 
 struct spmvhip_batch {
 	int tile = 0;        // units per work item, or 0: sized by item_nnz
 	long item_nnz = 16384;
 	int bnnz = 256; // LDS staging of the kernel instance (256 or 512)
+	int queues = 8; // 8: one queue per XCD; 1: every wave draws from one shared queue
 	std::vector<spmvhip_format *> fmts;
 	std::vector<spmvhip::BatchMat> host_mats;
 	std::vector<spmvhip::WorkItem> host_items;
+	std::vector<spmvhip::SplitRef> host_splits;
 	std::vector<long> mat_nnz;
 	std::vector<int> host_qstart;
-	spmvhip::DevBuf mats, items, qstart, counter;
+	spmvhip::DevBuf mats, items, qstart, counter, splits;
 	hipStream_t stream = nullptr;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
@@ -121,6 +158,11 @@ int batch_launch(spmvhip_batch *b, hipStream_t s)
 		else
 			batch_kernel<256><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->items.as<WorkItem>(),
 					b->qstart.as<int>(), b->counter.as<int>());
+		SPMVHIP_CHECK(hipGetLastError());
+	}
+	if (!b->host_splits.empty()) {
+		batch_split_finalize<<<grid_for((long) b->host_splits.size(), kBlock), kBlock, 0, s>>>(
+				b->mats.as<BatchMat>(), b->splits.as<SplitRef>(), (long) b->host_splits.size());
 		SPMVHIP_CHECK(hipGetLastError());
 	}
 	return SPMVHIP_OK;
@@ -151,6 +193,18 @@ int spmvhip_batch_create(int tile_units, spmvhip_batch **out)
 	return SPMVHIP_OK;
 }
 
+int spmvhip_batch_set_queues(spmvhip_batch *b, int nqueues)
+{
+	if (!b)
+		return set_error(SPMVHIP_EINVAL, "NULL batch");
+	if (b->finalized)
+		return set_error(SPMVHIP_EINVAL, "batch already finalized");
+	if (nqueues != 1 && nqueues != kXcds)
+		return set_error(SPMVHIP_EINVAL, "batch: %d queues (1 shared, or %d per XCD)", nqueues, kXcds);
+	b->queues = nqueues;
+	return SPMVHIP_OK;
+}
+
 int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d_x, double *d_y)
 {
 	if (!b || !f || !d_x || !d_y)
@@ -173,6 +227,13 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 		M.val = f->val.as<double>();
 		M.blocks = f->blocks.as<int32_t>();
 		M.lanes = f->lanes;
+		M.split_nnz = f->split_nnz;
+		if (f->nsegs > 0) {
+			M.seg_desc = f->seg_desc.as<int32_t>();
+			M.seg_ptr = f->seg_ptr.as<int32_t>();
+			M.split_rows = f->split_rows.as<int32_t>();
+			M.partials = f->partials.as<double>();
+		}
 		units = f->nblocks;
 	} else if (f->kind == SPMVHIP_SELL || f->kind == SPMVHIP_SELL_C_S) {
 		M.sptr = f->slice_ptr.as<int64_t>();
@@ -198,8 +259,18 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 	long per = b->tile;
 	if (per <= 0)
 		per = units > 0 ? std::max<long>(1, b->item_nnz * units / std::max<long>(1, f->nnz)) : 1;
+	// segments of split rows first (the longest work of the member), each
+	// item about item_nnz nonzeros; their rows are completed by
+	// batch_split_finalize after the persistent kernel
+	if (M.partials) {
+		const long nsegs = f->nsegs, sper = std::max<long>(1, b->item_nnz / std::max(1, f->split_nnz));
+		for (long u = 0; u < nsegs; u += sper)
+			b->host_items.push_back({mi, (int) u, (int) std::min(nsegs, u + sper), kItemSegments});
+		for (long i = 0; i < f->nsplit; i++)
+			b->host_splits.push_back({mi, (int) i});
+	}
 	for (long u = 0; u < units; u += per)
-		b->host_items.push_back({mi, (int) u, (int) std::min(units, u + per), 0});
+		b->host_items.push_back({mi, (int) u, (int) std::min(units, u + per), kItemUnits});
 	b->total_nnz += f->nnz;
 	return SPMVHIP_OK;
 }
@@ -219,7 +290,7 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 	std::stable_sort(by_size.begin(), by_size.end(), [&](int i, int j) { return b->mat_nnz[i] > b->mat_nnz[j]; });
 	long load[kXcds] = {0};
 	for (int i : by_size) {
-		const int q = (int) (std::min_element(load, load + kXcds) - load);
+		const int q = b->queues == 1 ? 0 : (int) (std::min_element(load, load + kXcds) - load);
 		queue_of[i] = q;
 		load[q] += b->mat_nnz[i] + 1;
 	}
@@ -245,6 +316,11 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 		SPMVHIP_CHECK(hipMemcpy(b->items.p, b->host_items.data(), b->host_items.size() * sizeof(WorkItem),
 				hipMemcpyHostToDevice));
 	SPMVHIP_CHECK(hipMemcpy(b->qstart.p, b->host_qstart.data(), (kXcds + 1) * sizeof(int), hipMemcpyHostToDevice));
+	if (!b->host_splits.empty()) {
+		SPMVHIP_TRY(b->splits.alloc(b->host_splits.size() * sizeof(SplitRef)));
+		SPMVHIP_CHECK(hipMemcpy(b->splits.p, b->host_splits.data(), b->host_splits.size() * sizeof(SplitRef),
+				hipMemcpyHostToDevice));
+	}
 	// persistent grid: as many workgroups as can be resident at once, a
 	// multiple of the XCD count so that every queue has the same number
 	int dev = 0, per_cu = 0;

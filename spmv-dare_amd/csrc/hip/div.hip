@@ -272,6 +272,78 @@ __device__ __forceinline__ uint64_t lds_le(const uint8_t *base, long pos, int nb
 	return nbytes >= 8 ? v : v & ((1ull << (8 * nbytes)) - 1);
 }
 
+// The same read from three aligned dwords and two byte-aligns (v_alignbyte
+// takes the 8 bytes {hi, lo} shifted right by a byte count): fewer
+// instructions than 64-bit shifts. pos is a packet offset below 2^31.
+__device__ __forceinline__ uint64_t lds_le_b32(const uint8_t *base, int pos, int nbytes)
+{
+	const uint32_t *w = (const uint32_t *) (base + (pos & ~3));
+	const uint32_t sh = (uint32_t) (pos & 3);
+	const uint32_t w0 = w[0], w1 = w[1], w2 = w[2];
+	uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, sh);
+	uint32_t hi = __builtin_amdgcn_alignbyte(w2, w1, sh);
+	lo &= nbytes >= 4 ? 0xffffffffu : (1u << (8 * nbytes)) - 1;
+	hi &= nbytes >= 8 ? 0xffffffffu : nbytes > 4 ? (1u << (8 * (nbytes - 4))) - 1 : 0u;
+	return ((uint64_t) hi << 32) | lo;
+}
+
+// At most 4 bytes (a coordinate of a packet with row_bits + col_bits <= 32).
+__device__ __forceinline__ uint32_t lds_le_b32x(const uint8_t *base, int pos, uint32_t mask)
+{
+	const uint32_t *w = (const uint32_t *) (base + (pos & ~3));
+	return __builtin_amdgcn_alignbyte(w[1], w[0], (uint32_t) (pos & 3)) & mask;
+}
+
+// One replication-factor-1 group of a packet staged in LDS (the common case:
+// every value distinct). The loads are unconditional: a step past the lane's
+// last value reads the packet's first coordinate and length byte (positions
+// inside the packet), gathers x[col_min] (a valid column) and decodes a zero
+// delta; only its LDS accumulation is skipped. C4: the coordinates fit 4
+// bytes and are read with one byte-align.
+template <int K, bool C4>
+__device__ __forceinline__ void div_rf1_group(const uint8_t *pk, int coords_at, int lens_at, int lens0, int crd0_at,
+		int stream_at, int steps, int cnt, int L, int cb, uint32_t col_min, int row_bits,
+		const double *__restrict__ x, double *yw, int &cursor)
+{
+	const uint32_t cmask = cb >= 4 ? 0xffffffffu : (1u << (8 * cb)) - 1;
+	const uint32_t rmask = (1u << row_bits) - 1; // row_bits <= 8 (at most 255 rows per packet)
+	uint64_t val = 0;
+	for (int p0 = 0; p0 < steps; p0 += K) {
+		uint32_t row[K];
+		double xv[K];
+		int len[K], tz[K];
+		bool ok[K];
+#pragma unroll
+		for (int k = 0; k < K; k++) {
+			ok[k] = p0 + k < cnt;
+			const int at = ok[k] ? crd0_at + (p0 + k) * L * cb : coords_at;
+			uint32_t col;
+			if (C4) {
+				const uint32_t c = lds_le_b32x(pk, at, cmask);
+				row[k] = c & rmask;
+				col = c >> row_bits;
+			} else {
+				const uint64_t c = lds_le_b32(pk, at, cb);
+				row[k] = (uint32_t) c & rmask;
+				col = (uint32_t) (c >> row_bits);
+			}
+			xv[k] = x[col_min + (ok[k] ? col : 0u)];
+			const int lb = pk[ok[k] ? lens0 + (p0 + k) * L : lens_at];
+			len[k] = ok[k] ? lb & 15 : 0;
+			tz[k] = (lb >> 4) << 2;
+		}
+#pragma unroll
+		for (int k = 0; k < K; k++) {
+			val += lds_le_b32(pk, stream_at + cursor, len[k]) << tz[k];
+			cursor += len[k];
+			double d;
+			__builtin_memcpy(&d, &val, 8);
+			if (ok[k]) // (an unconditional add of 0.0 would pile every idle lane onto one LDS address)
+				atomicAdd(&yw[row[k]], d * xv[k]);
+		}
+	}
+}
+
 template <int K>
 __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__restrict__ bytes,
 		const int64_t *__restrict__ pk_off, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
@@ -339,19 +411,21 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 			const long cnt = lane < L ? dv + (lane < md ? 1 : 0) : 0;
 			const long steps = dv + (md ? 1 : 0);
 			uint64_t val = 0;
+			if (rf == 1) {
+				int cur = (int) cursor;
+				const int lens0 = (int) (lens_at + ks) + lane, crd0 = (int) cs + lane;
+				if (cb <= 4)
+					div_rf1_group<K, true>(pk, (int) coords_at, (int) lens_at, lens0, (int) coords_at + crd0 * cb,
+							(int) stream_at, (int) steps, (int) cnt, L, cb, col_min, row_bits, x, yw, cur);
+				else
+					div_rf1_group<K, false>(pk, (int) coords_at, (int) lens_at, lens0, (int) coords_at + crd0 * cb,
+							(int) stream_at, (int) steps, (int) cnt, L, cb, col_min, row_bits, x, yw, cur);
+				cursor = cur;
+				ks += n;
+				cs += n;
+				continue;
+			}
 			for (long p0 = 0; p0 < steps; p0 += K) {
-				uint64_t c[K];
-				double xv[K];
-				if (rf == 1) {
-#pragma unroll
-					for (int k = 0; k < K; k++) {
-						const long pp = p0 + k;
-						c[k] = pp < cnt ? lds_le(pk, coords_at + (cs + pp * L + lane) * cb, cb) : 0;
-					}
-#pragma unroll
-					for (int k = 0; k < K; k++)
-						xv[k] = x[col_min + (uint32_t) (c[k] >> row_bits)];
-				}
 #pragma unroll
 				for (int k = 0; k < K; k++) {
 					const long pp = p0 + k;
@@ -365,9 +439,7 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 					}
 					double d;
 					__builtin_memcpy(&d, &val, 8);
-					if (rf == 1) {
-						atomicAdd(&yw[(uint32_t) (c[k] & row_mask)], d * xv[k]);
-					} else {
+					{
 						for (long t = 0; t < rf; t++) {
 							const long cpos = pp < dv ? (pp * rf + t) * L + lane : mul_c + lane * rf + t;
 							const uint64_t cc = lds_le(pk, coords_at + (cs + cpos) * cb, cb);

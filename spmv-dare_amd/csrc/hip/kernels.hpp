@@ -43,6 +43,21 @@ __device__ __forceinline__ void wave_lds_sync()
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// p[e] + p[e + L] + ... for e < b, with four LDS reads in flight per step.
+__device__ __forceinline__ double lds_strided_sum(const double *p, int e, int b, int L)
+{
+	double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+	for (; e + 3 * L < b; e += 4 * L) {
+		s0 += p[e];
+		s1 += p[e + L];
+		s2 += p[e + 2 * L];
+		s3 += p[e + 3 * L];
+	}
+	for (; e < b; e += L)
+		s0 += p[e];
+	return (s0 + s1) + (s2 + s3);
+}
+
 // y[r] = sum of row r over [j0, j1), one wave, lanes striding the row.
 __device__ __forceinline__ double wave_row_sum(const int32_t *__restrict__ col, const double *__restrict__ val,
 		const double *__restrict__ x, int j0, int j1)
@@ -130,10 +145,7 @@ __device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const 
 		const int a = __shfl(ra, r & (kWave - 1), kWave);
 		const int bn = __shfl(ra, (r + 1) & (kWave - 1), kWave);
 		const int b = r + 1 < kWave ? bn : nz;
-		double s = 0;
-		if (r < nrows)
-			for (int e = a + sub; e < b; e += L)
-				s += p[e];
+		double s = r < nrows ? lds_strided_sum(p, a + sub, b, L) : 0.0;
 		s = group_sum(s, L);
 		if (r < nrows && sub == 0)
 			y[r0 + r] = s;
@@ -238,10 +250,7 @@ __device__ __forceinline__ void csr_stage_finish(const CsrStage &st, const doubl
 		const int a = __shfl(st.ra, r & (kWave - 1), kWave);
 		const int bn = __shfl(st.ra, (r + 1) & (kWave - 1), kWave);
 		const int b = r + 1 < kWave ? bn : nz;
-		double s = 0;
-		if (r < nrows)
-			for (int e = a + sub; e < b; e += L)
-				s += p[e];
+		double s = r < nrows ? lds_strided_sum(p, a + sub, b, L) : 0.0;
 		s = group_sum(s, L);
 		if (r < nrows && sub == 0)
 			y[r0 + r] = s;

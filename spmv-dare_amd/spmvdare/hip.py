@@ -70,6 +70,7 @@ _sig = {
     "spmvhip_export": ([_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "spmvhip_destroy": ([_vp], None),
     "spmvhip_batch_create": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "spmvhip_batch_set_queues": ([_vp, ctypes.c_int], ctypes.c_int),
     "spmvhip_batch_add": ([_vp, _vp, _vp, _vp], ctypes.c_int),
     "spmvhip_batch_finalize": ([_vp, ctypes.c_int], ctypes.c_int),
     "spmvhip_batch_run": ([_vp], ctypes.c_int),
@@ -282,6 +283,10 @@ class Batch:
         _check(_lib.spmvhip_batch_create(tile_units, ctypes.byref(h)), "batch_create")
         self._h = h.value
         self._keep = []
+
+    def set_queues(self, nqueues):
+        """8 (default): one work queue per XCD; 1: one shared queue."""
+        _check(_lib.spmvhip_batch_set_queues(self._h, nqueues), "batch_set_queues")
 
     def add(self, fmt, dx, dy):
         _check(_lib.spmvhip_batch_add(self._h, fmt._h, dx.ptr, dy.ptr), "batch_add")

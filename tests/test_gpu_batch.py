@@ -94,6 +94,41 @@ def test_batch_with_block512_member(members):
     b.destroy()
 
 
+@pytest.mark.parametrize("use_graph,queues", [(False, 8), (True, 8), (True, 1)])
+def test_batch_split_rows(members, use_graph, queues):
+    """Members whose long rows are split into segments (split_nnz 256): the
+    segments run as work items of the persistent kernel and the finalize
+    pass adds each row's partials; y must match the oracle and the member's
+    own standalone launch (the same segments, the same order)."""
+    group = []
+    for i, name in enumerate(["dense_rows", "batch_03", "batch_07", "run_sh_1"]):
+        a = gen.generate(gen.GenParams.parse(GEN[name]["line"]), stats=False)
+        x = gen.make_x(a.n, 500 + i)
+        f = hip.csr_to_format(a.row_ptr, a.col_ind, a.values, a.m, a.n, a.nnz, kind="csr_vec", block_nnz=128,
+                              split_nnz=256)
+        y_alone = np.empty(a.m)
+        f.spmv(x, y_alone)
+        group.append(dict(name=name, kind="csr_vec_split", fmt=f, dx=hip.DeviceArray.from_host(x),
+                          dy=hip.DeviceArray(a.m), y_ref=pyoracle.csr_naive(a.row_ptr, a.col_ind, a.values, x),
+                          y_alone=y_alone, m=a.m, nsplit=f.info()["num_split_rows"]))
+    assert sum(g["nsplit"] for g in group) > 0
+    b = hip.Batch(0)
+    b.set_queues(queues)
+    for mb in group:
+        b.add(mb["fmt"], mb["dx"], mb["dy"])
+    b.finalize(use_graph=use_graph)
+    for rep in range(3):
+        for mb in group:
+            mb["dy"].upload(np.full(mb["m"], np.nan))
+        b.run()
+        b.synchronize()
+        for mb in group:
+            y = mb["dy"].download()
+            assert pyoracle.normwise_rel_err(y, mb["y_ref"]) <= 1e-13, mb["name"]
+            np.testing.assert_array_equal(y, mb["y_alone"])
+    b.destroy()
+
+
 def test_batch_rejects_unbatchable(members):
     a = gen.generate(gen.GenParams.parse(GEN["test_txt"]["line"]), stats=False)
     b = hip.Batch()
