@@ -177,17 +177,19 @@ void spmvhip_destroy(spmvhip_format *f);
 /* ---- batched SpMV (no counterpart in the reference, whose driver runs one
  * matrix per process, run.sh:116-151) ----
  * Many converted matrices, each with its own device x and y, computed by ONE
- * persistent kernel launch whose wavefronts draw (matrix, unit range) work
- * items from atomic tickets, one queue per XCD (matrices dealt to the queues
- * by nnz); with use_graph the launch (ticket reset + kernel) is captured once
- * into a hipGraph and replayed. Members must be CSR_VEC (block_nnz <= 512),
- * SELL or SELL_C_S handles, and must outlive the batch.
- * tile_units > 0: row blocks / slices per work item; 0: items of about 16384
- * nnz; < 0: items of about -tile_units nnz (at most 2^24). */
+ * persistent kernel launch: the resident wavefronts sweep the list of all
+ * members' units (CSR row blocks and long-row segments, SELL slices) in
+ * order, wave w taking units w, w + W, ...; the last units are work items
+ * drawn from atomic-ticket queues; long-row segments are completed by a
+ * second kernel of the same sequence. With use_graph the sequence (ticket
+ * reset + kernel + segment finalize) is captured once into a hipGraph and
+ * replayed. Members must be CSR_VEC (block_nnz <= 512), SELL or SELL_C_S
+ * handles, must outlive the batch and must not run elsewhere meanwhile.
+ * tile_units: units per work item of the ticketed tail (0 = 4, at most 65536). */
 typedef struct spmvhip_batch spmvhip_batch;
 int spmvhip_batch_create(int tile_units, spmvhip_batch **out);
-/* Work queues (before finalize): 8 (default) = one per XCD, matrices dealt to
- * them by nnz; 1 = one shared queue in insertion order. */
+/* Tail queues (before finalize): 8 (default) = one per XCD, items dealt
+ * round-robin; 1 = one shared queue. */
 int spmvhip_batch_set_queues(spmvhip_batch *b, int nqueues);
 int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d_x, double *d_y);
 int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph);
@@ -196,6 +198,7 @@ int spmvhip_batch_synchronize(spmvhip_batch *b);
 /* ms[i]: time of iteration i of the batched launch (per_matrix = 0) or of
  * one standalone spmv launch per member, back to back (per_matrix = 1). */
 int spmvhip_batch_time(spmvhip_batch *b, int per_matrix, int warmup, int iters, float *ms);
+/* nitems: units of the batch (row blocks, segments, slices); grid: workgroups of the launch. */
 int spmvhip_batch_info(const spmvhip_batch *b, long *nmats, long *nitems, int *grid, long *total_nnz);
 void spmvhip_batch_destroy(spmvhip_batch *b);
 

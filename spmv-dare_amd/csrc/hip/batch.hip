@@ -1,27 +1,32 @@
 // Batched SpMV: many independent converted matrices, each with its own x and
-// y, computed by ONE persistent kernel launch that takes work from queues
-// (atomic tickets over (matrix, unit range) items), optionally replayed from
-// a captured hipGraph. The reference has no counterpart: its driver runs one
+// y, computed by ONE persistent kernel launch, optionally replayed from a
+// captured hipGraph. The reference has no counterpart: its driver runs one
 // matrix per process (run.sh:116-151).
 //
 // Units are the formats' own per-wave work: a row block of the binned CSR
-// format or a 64-row slice of SELL / SELL-C-sigma (kernels.hpp). A work item
-// is consecutive units of one matrix (about 16k nonzeros, or `tile` units),
-// taken by one wavefront. CSR rows longer than the member's split_nnz are
-// cut into the standalone kernel's segments: segment work items write
-// partial sums, and batch_split_finalize (the next node of the same launch
-// sequence) adds each row's partials in segment order, so a heavy-tailed row
-// is spread over many waves instead of holding one.
+// format, a 64-row slice of SELL / SELL-C-sigma (kernels.hpp), or a segment
+// of a CSR row longer than the member's split_nnz (the standalone kernel's
+// segments: a partial sum per segment, and batch_split_finalize, the next
+// node of the same launch sequence, adds each row's partials in segment
+// order, so a heavy-tailed row is spread over many waves).
 //
-// XCD-aware scheduling: the matrices are dealt to 8 queues, one per XCD,
-// balanced by nnz (largest first to the least loaded queue). Workgroup b is
-// dispatched to XCD b % 8 and first drains queue b % 8, so each XCD's L2
-// holds the x vectors of its own matrices only; a wave whose queue is empty
-// moves on to the next queues, which balances the tail. Correctness does not
-// depend on the dispatch order, only locality does.
+// Schedule. All units of all members form one list (members in insertion
+// order, each member's segments before its other units). The resident waves
+// (W of them) first sweep the list statically: wave w takes units w, w + W,
+// w + 2W, ..., so at any time the waves work on a window of about W adjacent
+// units -- the same in-order locality (x lines in L2, open DRAM pages) that
+// hardware dispatch gives a one-wave-per-unit launch, with no per-unit
+// bookkeeping. The last units (the tail, where per-wave differences in unit
+// cost would leave waves idle) form small work items on queues drawn from by
+// atomic tickets: one queue per XCD (workgroup b runs on XCD b % 8 and draws
+// from queue b % 8 only; the items are dealt round-robin, so the queues hold
+// the same work to within one item), or one shared queue. The tickets are
+// device-scope atomics, performed at the memory side: each counter has a
+// 256-byte line of its own, and a wave takes exactly one failing ticket, so
+// the launch's end does not queue tens of thousands of atomics on one line.
+// Correctness does not depend on the dispatch order, only locality does.
 
 #include <algorithm>
-#include <numeric>
 #include <vector>
 
 #include "format.hpp"
@@ -30,29 +35,31 @@
 namespace spmvhip {
 
 constexpr int kBatchBnnzMax = 512; // largest csr_vec block_nnz a member may have
-constexpr long kItemNnz = 16384;   // default work-item size in nnz (one ticket each)
+constexpr int kTailItemUnits = 4;  // default units per work item of the dynamic tail
+constexpr int kTailRounds = 4;     // the tail is the last kTailRounds * W units
+constexpr int kCounterStride = 64; // ints between queue counters: one 256-byte line each
 
+// A member's device arrays, as pointer values (common.hpp: global_ptr).
 struct BatchMat {
 	int kind, lanes;
 	long m;
-	const int32_t *rp, *col, *blocks;
-	const double *val;
-	const int64_t *sptr;
-	const int32_t *sw, *scol, *order;
-	const double *sval;
-	const double *x;
-	double *y;
+	uint64_t rp, col, blocks, val;         // int32, int32, int32, double (CSR_VEC)
+	uint64_t sptr, sw, scol, order, sval;  // int64, int32, int32, int32 (0 for SELL), double (SELL*)
+	uint64_t x, y;                         // double
 	// CSR rows longer than split_nnz: segments (row, j0, j1) summed into
-	// partials by the persistent kernel, added per row by batch_split_finalize
-	const int32_t *seg_desc, *seg_ptr, *split_rows;
-	double *partials;
+	// partials, added per row by batch_split_finalize
+	uint64_t seg_desc, seg_ptr, split_rows, partials; // int32 x 3, double (0 without split rows)
 	int split_nnz;
 };
 
-enum { kItemUnits = 0, kItemSegments = 1 };
+// one unit of the global list: member `mat`, its unit u >= 0 (row block or
+// slice) or its split-row segment -u - 1
+struct UnitRef {
+	int mat, u;
+};
 
 struct WorkItem {
-	int mat, u0, u1, type; // units (row blocks / slices) or split-row segments [u0, u1)
+	int g0, g1; // a range of the global unit list
 };
 
 struct SplitRef {
@@ -64,44 +71,68 @@ struct SplitRef {
 __global__ void batch_reset(int *counter)
 {
 	if (threadIdx.x < kXcds)
-		counter[threadIdx.x] = 0;
+		counter[threadIdx.x * kCounterStride] = 0;
 }
 
-// items[qstart[q] .. qstart[q + 1]) is queue q. BNNZ: LDS staging per wave
-// for CSR row blocks, at least the largest member block_nnz.
+template <int BNNZ>
+__device__ __forceinline__ void batch_unit(const BatchMat &M, int u, double *prod)
+{
+	const double *x = global_ptr<const double>(M.x);
+	double *y = global_ptr<double>(M.y);
+	if (u < 0) {
+		const int s = -u - 1;
+		const int32_t *desc = const_ptr<int32_t>(M.seg_desc);
+		const double sum = wave_row_sum(global_ptr<const int32_t>(M.col), global_ptr<const double>(M.val), x,
+				desc[3 * s + 1], desc[3 * s + 2]);
+		if (lane_id() == 0)
+			global_ptr<double>(M.partials)[s] = sum;
+	} else if (M.kind == SPMVHIP_CSR_VEC) {
+		const int32_t *blocks = const_ptr<int32_t>(M.blocks);
+		csr_block<BNNZ>(const_ptr<int32_t>(M.rp), global_ptr<const int32_t>(M.col),
+				global_ptr<const double>(M.val), x, y, blocks[u], blocks[u + 1], M.split_nnz, M.partials != 0,
+				M.lanes, prod);
+	} else {
+		sell_slice<4>(const_ptr<int64_t>(M.sptr), const_ptr<int32_t>(M.sw),
+				global_ptr<const int32_t>(M.scol), global_ptr<const double>(M.sval),
+				M.order ? global_ptr<const int32_t>(M.order) : nullptr, x, y, M.m, u, M.lanes);
+	}
+}
+
+// units[0, nstatic): static sweep; items[qstart[q] .. qstart[q + 1]) is tail
+// queue q. BNNZ: LDS staging per wave for CSR row blocks, at least the
+// largest member block_nnz.
+//
+// Unit indices are 32-bit (finalize checks the count): the loop bound is then
+// compared on the scalar unit. A 64-bit signed compare has no scalar form, and
+// the VGPRs it borrowed at the loop latch made the compiler wait there for
+// every outstanding memory operation of the wave, the y stores included.
 template <int BNNZ>
 __global__ void __launch_bounds__(kBlock) batch_kernel(const BatchMat *__restrict__ mats,
-		const WorkItem *__restrict__ items, const int *__restrict__ qstart, int *__restrict__ counter)
+		const UnitRef *__restrict__ units, int nstatic, const WorkItem *__restrict__ items,
+		const int *__restrict__ qstart, int nqueues, int *__restrict__ counter)
 {
 	__shared__ double prod[kWavesPerBlock][2 * BNNZ];
-	const int wv = threadIdx.x / kWave;
-	const int home = (int) (blockIdx.x % kXcds);
-	for (int k = 0; k < kXcds; k++) {
-		const int q = (home + k) % kXcds;
-		const int q0 = qstart[q], q1 = qstart[q + 1];
-		for (;;) {
-			int t = 0;
-			if (lane_id() == 0)
-				t = atomicAdd(counter + q, 1);
-			t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, kWave));
-			if (q0 + t >= q1)
-				break; // queue drained: wave-uniform exit, every wave reaches it
-			const WorkItem wi = items[q0 + t];
-			const BatchMat &M = mats[wi.mat];
-			if (wi.type == kItemSegments) {
-				for (int u = wi.u0; u < wi.u1; u++) {
-					const double sum = wave_row_sum(M.col, M.val, M.x, M.seg_desc[3 * u + 1], M.seg_desc[3 * u + 2]);
-					if (lane_id() == 0)
-						M.partials[u] = sum;
-				}
-			} else if (M.kind == SPMVHIP_CSR_VEC) {
-				for (int u = wi.u0; u < wi.u1; u++)
-					csr_block<BNNZ>(M.rp, M.col, M.val, M.x, M.y, M.blocks[u], M.blocks[u + 1], M.split_nnz,
-							M.partials != nullptr, M.lanes, prod[wv]);
-			} else {
-				for (int u = wi.u0; u < wi.u1; u++)
-					sell_slice<4>(M.sptr, M.sw, M.scol, M.sval, M.order, M.x, M.y, M.m, u, M.lanes);
-			}
+	const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+	const int W = (int) gridDim.x * kWavesPerBlock;
+	for (int g = (int) blockIdx.x * kWavesPerBlock + wv; g < nstatic; g += W) {
+		const UnitRef r = units[g];
+		batch_unit<BNNZ>(mats[r.mat], r.u, prod[wv]);
+	}
+	const int q = (int) (blockIdx.x % nqueues);
+	const int q0 = qstart[q], q1 = qstart[q + 1];
+	if (q0 == q1)
+		return;
+	for (;;) {
+		int t = 0;
+		if (lane_id() == 0)
+			t = atomicAdd(counter + q * kCounterStride, 1);
+		t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, kWave));
+		if (q0 + t >= q1)
+			break; // queue drained: wave-uniform exit, every wave reaches it
+		const WorkItem wi = items[q0 + t];
+		for (int g = wi.g0; g < wi.g1; g++) {
+			const UnitRef r = units[g];
+			batch_unit<BNNZ>(mats[r.mat], r.u, prod[wv]);
 		}
 	}
 }
@@ -116,25 +147,27 @@ __global__ void batch_split_finalize(const BatchMat *__restrict__ mats, const Sp
 	const BatchMat &M = mats[refs[k].mat];
 	const int i = refs[k].i;
 	double s = 0;
-	for (int j = M.seg_ptr[i]; j < M.seg_ptr[i + 1]; j++)
-		s += M.partials[j];
-	M.y[M.split_rows[i]] = s;
+	const int32_t *seg_ptr = global_ptr<const int32_t>(M.seg_ptr);
+	const double *partials = global_ptr<const double>(M.partials);
+	for (int j = seg_ptr[i]; j < seg_ptr[i + 1]; j++)
+		s += partials[j];
+	global_ptr<double>(M.y)[global_ptr<const int32_t>(M.split_rows)[i]] = s;
 }
 
 } // namespace spmvhip
 
 struct spmvhip_batch {
-	int tile = 0;        // units per work item, or 0: sized by item_nnz
-	long item_nnz = 16384;
-	int bnnz = 256; // LDS staging of the kernel instance (256 or 512)
-	int queues = 8; // 8: one queue per XCD; 1: every wave draws from one shared queue
+	int tile = spmvhip::kTailItemUnits; // units per work item of the dynamic tail
+	int bnnz = 256;                     // LDS staging of the kernel instance (256 or 512)
+	int queues = 8;                     // tail queues: 8 = one per XCD, 1 = shared
 	std::vector<spmvhip_format *> fmts;
 	std::vector<spmvhip::BatchMat> host_mats;
+	std::vector<spmvhip::UnitRef> host_units;
 	std::vector<spmvhip::WorkItem> host_items;
 	std::vector<spmvhip::SplitRef> host_splits;
-	std::vector<long> mat_nnz;
 	std::vector<int> host_qstart;
-	spmvhip::DevBuf mats, items, qstart, counter, splits;
+	int nstatic = 0;
+	spmvhip::DevBuf mats, units, items, qstart, counter, splits;
 	hipStream_t stream = nullptr;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
@@ -151,13 +184,13 @@ int batch_launch(spmvhip_batch *b, hipStream_t s)
 {
 	batch_reset<<<1, kWave, 0, s>>>(b->counter.as<int>());
 	SPMVHIP_CHECK(hipGetLastError());
-	if (!b->host_items.empty()) {
+	if (!b->host_units.empty()) {
 		if (b->bnnz > 256)
-			batch_kernel<kBatchBnnzMax><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->items.as<WorkItem>(),
-					b->qstart.as<int>(), b->counter.as<int>());
+			batch_kernel<kBatchBnnzMax><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->units.as<UnitRef>(),
+					b->nstatic, b->items.as<WorkItem>(), b->qstart.as<int>(), b->queues, b->counter.as<int>());
 		else
-			batch_kernel<256><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->items.as<WorkItem>(),
-					b->qstart.as<int>(), b->counter.as<int>());
+			batch_kernel<256><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->units.as<UnitRef>(),
+					b->nstatic, b->items.as<WorkItem>(), b->qstart.as<int>(), b->queues, b->counter.as<int>());
 		SPMVHIP_CHECK(hipGetLastError());
 	}
 	if (!b->host_splits.empty()) {
@@ -177,13 +210,13 @@ int spmvhip_batch_create(int tile_units, spmvhip_batch **out)
 	if (!out)
 		return set_error(SPMVHIP_EINVAL, "out is NULL");
 	*out = nullptr;
-	if (tile_units < -(1 << 24) || tile_units > 65536)
-		return set_error(SPMVHIP_EINVAL, "tile_units %d", tile_units);
+	if (tile_units < 0 || tile_units > 65536)
+		return set_error(SPMVHIP_EINVAL, "tile_units %d (0..65536)", tile_units);
 	spmvhip_batch *b = new (std::nothrow) spmvhip_batch();
 	if (!b)
 		return set_error(SPMVHIP_ENOMEM, "batch handle");
-	b->tile = tile_units > 0 ? tile_units : 0;
-	b->item_nnz = tile_units < 0 ? -(long) tile_units : kItemNnz;
+	if (tile_units > 0)
+		b->tile = tile_units;
 	hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
 	if (e != hipSuccess) {
 		delete b;
@@ -214,63 +247,52 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 	BatchMat M{};
 	M.kind = f->kind;
 	M.m = f->m;
-	M.x = d_x;
-	M.y = d_y;
-	long units;
+	M.x = ptr_bits(d_x);
+	M.y = ptr_bits(d_y);
+	long units, nsegs = 0;
 	if (f->kind == SPMVHIP_CSR_VEC) {
 		if (f->block_nnz > kBatchBnnzMax)
 			return set_error(SPMVHIP_EINVAL, "batch: csr_vec block_nnz %d > %d", f->block_nnz, kBatchBnnzMax);
 		if (f->block_nnz > b->bnnz)
 			b->bnnz = kBatchBnnzMax;
-		M.rp = f->row_ptr.as<int32_t>();
-		M.col = f->col.as<int32_t>();
-		M.val = f->val.as<double>();
-		M.blocks = f->blocks.as<int32_t>();
+		M.rp = ptr_bits(f->row_ptr.p);
+		M.col = ptr_bits(f->col.p);
+		M.val = ptr_bits(f->val.p);
+		M.blocks = ptr_bits(f->blocks.p);
 		M.lanes = f->lanes;
 		M.split_nnz = f->split_nnz;
 		if (f->nsegs > 0) {
-			M.seg_desc = f->seg_desc.as<int32_t>();
-			M.seg_ptr = f->seg_ptr.as<int32_t>();
-			M.split_rows = f->split_rows.as<int32_t>();
-			M.partials = f->partials.as<double>();
+			M.seg_desc = ptr_bits(f->seg_desc.p);
+			M.seg_ptr = ptr_bits(f->seg_ptr.p);
+			M.split_rows = ptr_bits(f->split_rows.p);
+			M.partials = ptr_bits(f->partials.p);
+			nsegs = f->nsegs;
 		}
 		units = f->nblocks;
 	} else if (f->kind == SPMVHIP_SELL || f->kind == SPMVHIP_SELL_C_S) {
-		M.sptr = f->slice_ptr.as<int64_t>();
-		M.sw = f->slice_w.as<int32_t>();
-		M.scol = f->s_cols.as<int32_t>();
-		M.sval = f->s_vals.as<double>();
-		M.order = f->kind == SPMVHIP_SELL_C_S ? f->row_order.as<int32_t>() : nullptr;
+		M.sptr = ptr_bits(f->slice_ptr.p);
+		M.sw = ptr_bits(f->slice_w.p);
+		M.scol = ptr_bits(f->s_cols.p);
+		M.sval = ptr_bits(f->s_vals.p);
+		M.order = f->kind == SPMVHIP_SELL_C_S ? ptr_bits(f->row_order.p) : 0;
 		M.lanes = f->lanes; // lanes per row T (>= 1) of the slice layout
 		units = f->nslices;
 	} else {
 		return set_error(SPMVHIP_EINVAL, "batch: format %s is not batchable (csr_vec, sell, sell_c_s are)",
 				f->name);
 	}
-	if (units > INT32_MAX)
+	if (units > INT32_MAX || nsegs > INT32_MAX || b->host_mats.size() >= (size_t) INT32_MAX)
 		return set_error(SPMVHIP_EINVAL, "batch: too many units");
-	int mi = (int) b->host_mats.size();
+	const int mi = (int) b->host_mats.size();
 	b->host_mats.push_back(M);
-	b->mat_nnz.push_back(f->nnz);
 	b->fmts.push_back(const_cast<spmvhip_format *>(f));
-	// units per item: fixed, or enough units of this matrix's average size
-	// for about item_nnz nonzeros (a device-scope ticket costs as much as
-	// streaming several KB, so items are kept coarse)
-	long per = b->tile;
-	if (per <= 0)
-		per = units > 0 ? std::max<long>(1, b->item_nnz * units / std::max<long>(1, f->nnz)) : 1;
-	// segments of split rows first (the longest work of the member), each
-	// item about item_nnz nonzeros; their rows are completed by
-	// batch_split_finalize after the persistent kernel
-	if (M.partials) {
-		const long nsegs = f->nsegs, sper = std::max<long>(1, b->item_nnz / std::max(1, f->split_nnz));
-		for (long u = 0; u < nsegs; u += sper)
-			b->host_items.push_back({mi, (int) u, (int) std::min(nsegs, u + sper), kItemSegments});
-		for (long i = 0; i < f->nsplit; i++)
-			b->host_splits.push_back({mi, (int) i});
-	}
-	for (long u = 0; u < units; u += per)
-		b->host_items.push_back({mi, (int) u, (int) std::min(units, u + per), kItemUnits});
+	// the member's segments (its longest work) first, then its units
+	for (long s = 0; s < nsegs; s++)
+		b->host_units.push_back({mi, (int) (-s - 1)});
+	for (long i = 0; i < f->nsplit && nsegs > 0; i++)
+		b->host_splits.push_back({mi, (int) i});
+	for (long u = 0; u < units; u++)
+		b->host_units.push_back({mi, (int) u});
 	b->total_nnz += f->nnz;
 	return SPMVHIP_OK;
 }
@@ -281,37 +303,46 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 		return set_error(SPMVHIP_EINVAL, "NULL batch");
 	if (b->finalized)
 		return set_error(SPMVHIP_EINVAL, "batch already finalized");
-	if (b->host_items.size() > (size_t) INT32_MAX)
-		return set_error(SPMVHIP_EINVAL, "batch: too many work items");
-	// deal the matrices to the XCD queues: largest first to the least loaded
-	const size_t nm = b->host_mats.size();
-	std::vector<int> by_size(nm), queue_of(nm);
-	std::iota(by_size.begin(), by_size.end(), 0);
-	std::stable_sort(by_size.begin(), by_size.end(), [&](int i, int j) { return b->mat_nnz[i] > b->mat_nnz[j]; });
-	long load[kXcds] = {0};
-	for (int i : by_size) {
-		const int q = b->queues == 1 ? 0 : (int) (std::min_element(load, load + kXcds) - load);
-		queue_of[i] = q;
-		load[q] += b->mat_nnz[i] + 1;
-	}
-	std::vector<WorkItem> sorted;
-	sorted.reserve(b->host_items.size());
+	// persistent grid: as many workgroups as can be resident at once, a
+	// multiple of the XCD count so that every tail queue has the same number
+	int dev = 0, cus = 0, per_cu = 0;
+	SPMVHIP_CHECK(hipGetDevice(&dev));
+	SPMVHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+	if (b->bnnz > 256)
+		SPMVHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<kBatchBnnzMax>, kBlock, 0));
+	else
+		SPMVHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<256>, kBlock, 0));
+	long g = (long) cus * std::max(per_cu, 1);
+	g = std::max<long>(kXcds, g / kXcds * kXcds);
+	b->grid = (int) g;
+	// the static sweep takes all but the last kTailRounds * W units; the tail
+	// becomes items of `tile` units dealt round-robin to the queues
+	const long N = (long) b->host_units.size(), W = g * kWavesPerBlock;
+	if (N + W > INT32_MAX)
+		return set_error(SPMVHIP_EINVAL, "batch: %ld work units (the kernel indexes them with 32 bits)", N);
+	b->nstatic = (int) std::max<long>(0, N - kTailRounds * W);
+	std::vector<std::vector<WorkItem>> q(kXcds);
+	long k = 0;
+	for (long g0 = b->nstatic; g0 < N; g0 += b->tile, k++)
+		q[b->queues == 1 ? 0 : k % kXcds].push_back({(int) g0, (int) std::min(N, g0 + b->tile)});
+	b->host_items.clear();
 	b->host_qstart.assign(kXcds + 1, 0);
-	for (int q = 0; q < kXcds; q++) {
-		b->host_qstart[q] = (int) sorted.size();
-		for (const WorkItem &w : b->host_items)
-			if (queue_of[w.mat] == q)
-				sorted.push_back(w);
+	for (int i = 0; i < kXcds; i++) {
+		b->host_qstart[i] = (int) b->host_items.size();
+		b->host_items.insert(b->host_items.end(), q[i].begin(), q[i].end());
 	}
-	b->host_qstart[kXcds] = (int) sorted.size();
-	b->host_items.swap(sorted);
+	b->host_qstart[kXcds] = (int) b->host_items.size();
 
+	const size_t nm = b->host_mats.size();
 	SPMVHIP_TRY(b->mats.alloc(std::max<size_t>(1, nm) * sizeof(BatchMat)));
+	SPMVHIP_TRY(b->units.alloc(std::max<size_t>(1, b->host_units.size()) * sizeof(UnitRef)));
 	SPMVHIP_TRY(b->items.alloc(std::max<size_t>(1, b->host_items.size()) * sizeof(WorkItem)));
 	SPMVHIP_TRY(b->qstart.alloc((kXcds + 1) * sizeof(int)));
-	SPMVHIP_TRY(b->counter.alloc(kXcds * sizeof(int)));
+	SPMVHIP_TRY(b->counter.alloc(kXcds * kCounterStride * sizeof(int)));
 	if (nm)
 		SPMVHIP_CHECK(hipMemcpy(b->mats.p, b->host_mats.data(), nm * sizeof(BatchMat), hipMemcpyHostToDevice));
+	if (N)
+		SPMVHIP_CHECK(hipMemcpy(b->units.p, b->host_units.data(), N * sizeof(UnitRef), hipMemcpyHostToDevice));
 	if (!b->host_items.empty())
 		SPMVHIP_CHECK(hipMemcpy(b->items.p, b->host_items.data(), b->host_items.size() * sizeof(WorkItem),
 				hipMemcpyHostToDevice));
@@ -321,19 +352,6 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 		SPMVHIP_CHECK(hipMemcpy(b->splits.p, b->host_splits.data(), b->host_splits.size() * sizeof(SplitRef),
 				hipMemcpyHostToDevice));
 	}
-	// persistent grid: as many workgroups as can be resident at once, a
-	// multiple of the XCD count so that every queue has the same number
-	int dev = 0, per_cu = 0;
-	hipDeviceProp_t prop;
-	SPMVHIP_CHECK(hipGetDevice(&dev));
-	SPMVHIP_CHECK(hipGetDeviceProperties(&prop, dev));
-	if (b->bnnz > 256)
-		SPMVHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<kBatchBnnzMax>, kBlock, 0));
-	else
-		SPMVHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<256>, kBlock, 0));
-	long g = (long) prop.multiProcessorCount * std::max(per_cu, 1);
-	g = std::max<long>(kXcds, g / kXcds * kXcds);
-	b->grid = (int) g;
 	b->finalized = true;
 	if (use_graph) {
 		SPMVHIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -377,7 +395,8 @@ int spmvhip_batch_time(spmvhip_batch *b, int per_matrix, int warmup, int iters, 
 		if (!per_matrix)
 			return spmvhip_batch_run(b);
 		for (size_t i = 0; i < b->fmts.size(); i++)
-			SPMVHIP_TRY(spmvhip_spmv_device(b->fmts[i], b->host_mats[i].x, b->host_mats[i].y, b->stream));
+			SPMVHIP_TRY(spmvhip_spmv_device(b->fmts[i], (const double *) (uintptr_t) b->host_mats[i].x,
+					(double *) (uintptr_t) b->host_mats[i].y, b->stream));
 		return SPMVHIP_OK;
 	};
 	for (int i = 0; i < warmup; i++)
@@ -414,7 +433,7 @@ int spmvhip_batch_info(const spmvhip_batch *b, long *nmats, long *nitems, int *g
 	if (nmats)
 		*nmats = (long) b->host_mats.size();
 	if (nitems)
-		*nitems = (long) b->host_items.size();
+		*nitems = (long) b->host_units.size();
 	if (grid)
 		*grid = b->grid;
 	if (total_nnz)

@@ -62,6 +62,36 @@ __device__ __forceinline__ int lane_id()
 	return (int) (threadIdx.x & (kWave - 1));
 }
 
+// A device pointer kept in a table (not passed as a kernel argument) is
+// stored as its integer value and materialised here directly in the global
+// address space: accesses through a generic pointer loaded from memory would
+// compile to flat instructions, which also count against the LDS wait
+// counter and so serialise LDS work behind outstanding memory loads.
+template <typename T>
+__device__ __forceinline__ T *global_ptr(uint64_t bits)
+{
+	return (T *) (__attribute__((address_space(1))) T *) bits;
+}
+
+// The same for a read-only table, in the constant address space: a load at a
+// wave-uniform address compiles to a scalar load (counted on the LDS/scalar
+// wait counter), a divergent one to a global load. Through a global pointer a
+// uniform load is a vector load whenever the compiler cannot prove that no
+// store of the kernel reaches it (any persistent loop that writes y), and a
+// vector load's wait also waits for every earlier store of the wave, so the
+// next work unit's bounds would only arrive after the previous unit's y
+// stores were acknowledged.
+template <typename T>
+__device__ __forceinline__ const T *const_ptr(uint64_t bits)
+{
+	return (const T *) (__attribute__((address_space(4))) const T *) bits;
+}
+
+__host__ __device__ inline uint64_t ptr_bits(const void *p)
+{
+	return (uint64_t) (uintptr_t) p;
+}
+
 // Owned device allocation (freed with the owning handle).
 struct DevBuf {
 	void *p = nullptr;

@@ -216,7 +216,8 @@ __global__ void __launch_bounds__(kBlock) csr_binned_kernel(const int32_t *__res
 // consecutive blocks), with block w + W's stream requested while block w is
 // reduced (kernels.hpp: csr_stage_*).
 template <int BNNZ_MAX>
-__global__ void __launch_bounds__(kBlock) csr_binned_pipe_kernel(const int32_t *__restrict__ rp,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
+csr_binned_pipe_kernel(const int32_t *__restrict__ rp,
 		const int32_t *__restrict__ col, const double *__restrict__ val, const double *__restrict__ x,
 		double *__restrict__ y, const int32_t *__restrict__ blocks, long nblocks, int split_nnz, int lanes_cfg)
 {
@@ -226,6 +227,10 @@ __global__ void __launch_bounds__(kBlock) csr_binned_pipe_kernel(const int32_t *
 	long w = (long) blockIdx.x * kWavesPerBlock + wv;
 	if (w >= nblocks)
 		return; // waves never synchronise with each other: early exit is safe
+	// block bounds and row starts through the constant address space: scalar
+	// loads, which do not wait for the previous block's gathers and stores
+	rp = const_ptr<int32_t>(ptr_bits(rp));
+	blocks = const_ptr<int32_t>(ptr_bits(blocks));
 	CsrStage cur, nxt;
 	csr_stage_load<BNNZ_MAX>(cur, rp, col, val, blocks[w], blocks[w + 1]);
 	for (;;) {

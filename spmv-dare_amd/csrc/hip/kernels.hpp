@@ -74,6 +74,43 @@ __device__ __forceinline__ double wave_row_sum(const int32_t *__restrict__ col, 
 	return wave_sum(a0 + a1);
 }
 
+// Second phase of a CSR row block: p[0, nz) holds the block's products in
+// entry order, lane i's `ra` the start of row i relative to the block
+// (rows past the block hold nz); yb[r] = sum of row r for r < nrows.
+// Lanes per row L = 2^lg: the largest power of two <= 64 / nrows (or the
+// configured value); groups of L lanes take rows g, g + 64/L, ...
+//
+// The first row's bounds are shuffled before the (divergent) row loop, so
+// that every path through the block consumes the row-start load: a load
+// left pending on some path makes the compiler wait for all of the wave's
+// memory operations, its y stores included, at the next reuse of the
+// load's register -- in a persistent loop, at the start of the next block.
+__device__ __forceinline__ void block_rows_reduce(const double *p, int ra, int nrows, int nz, int lanes_cfg,
+		double *__restrict__ yb)
+{
+	const int lane = lane_id();
+	int lg = 6;
+	while (lg > 0 && (nrows << lg) > kWave)
+		lg--;
+	if (lanes_cfg > 0 && lanes_cfg < (1 << lg))
+		lg = 31 - __builtin_clz(lanes_cfg);
+	const int L = 1 << lg, g = lane >> lg, sub = lane & (L - 1), gshift = 6 - lg;
+	const int rows_up = (nrows + (1 << gshift) - 1) >> gshift << gshift;
+	int an = __shfl(ra, g, kWave), bnn = __shfl(ra, (g + 1) & (kWave - 1), kWave);
+	for (int r = g; r < rows_up; r += 1 << gshift) {
+		const int a = an, bn = bnn;
+		const int rn = r + (1 << gshift);
+		an = __shfl(ra, rn & (kWave - 1), kWave);
+		bnn = __shfl(ra, (rn + 1) & (kWave - 1), kWave);
+		const int b = r + 1 < kWave ? bn : nz;
+		double s = r < nrows ? lds_strided_sum(p, a + sub, b, L) : 0.0;
+		s = group_sum(s, L);
+		if (r < nrows && sub == 0)
+			yb[r] = s;
+	}
+	wave_lds_sync(); // the buffer is reused by the wave's next block
+}
+
 // One row block [r0, r1) of the binned CSR format (see csr.hip), processed
 // by the calling wavefront. `p` is the wave's LDS buffer of 2 * BNNZ_MAX
 // doubles. A single row longer than BNNZ_MAX is summed by the whole wave;
@@ -132,25 +169,7 @@ __device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const 
 		}
 	}
 	wave_lds_sync();
-	// lanes per row L = 2^lg: largest power of two <= 64 / nrows (or the
-	// configured value); groups of L lanes take rows g, g + 64/L, ...
-	int lg = 6;
-	while (lg > 0 && (nrows << lg) > kWave)
-		lg--;
-	if (lanes_cfg > 0 && lanes_cfg < (1 << lg))
-		lg = 31 - __builtin_clz(lanes_cfg);
-	const int L = 1 << lg, g = lane >> lg, sub = lane & (L - 1), gshift = 6 - lg;
-	const int rows_up = (nrows + (1 << gshift) - 1) >> gshift << gshift;
-	for (int r = g; r < rows_up; r += 1 << gshift) {
-		const int a = __shfl(ra, r & (kWave - 1), kWave);
-		const int bn = __shfl(ra, (r + 1) & (kWave - 1), kWave);
-		const int b = r + 1 < kWave ? bn : nz;
-		double s = r < nrows ? lds_strided_sum(p, a + sub, b, L) : 0.0;
-		s = group_sum(s, L);
-		if (r < nrows && sub == 0)
-			y[r0 + r] = s;
-	}
-	wave_lds_sync(); // the buffer is reused by the wave's next block
+	block_rows_reduce(p, ra, nrows, nz, lanes_cfg, y + r0);
 }
 
 // The same row block split in three stages, so that a persistent wave can
@@ -239,23 +258,7 @@ __device__ __forceinline__ void csr_stage_finish(const CsrStage &st, const doubl
 		}
 	}
 	wave_lds_sync();
-	int lg = 6;
-	while (lg > 0 && (nrows << lg) > kWave)
-		lg--;
-	if (lanes_cfg > 0 && lanes_cfg < (1 << lg))
-		lg = 31 - __builtin_clz(lanes_cfg);
-	const int L = 1 << lg, g = lane >> lg, sub = lane & (L - 1), gshift = 6 - lg;
-	const int rows_up = (nrows + (1 << gshift) - 1) >> gshift << gshift;
-	for (int r = g; r < rows_up; r += 1 << gshift) {
-		const int a = __shfl(st.ra, r & (kWave - 1), kWave);
-		const int bn = __shfl(st.ra, (r + 1) & (kWave - 1), kWave);
-		const int b = r + 1 < kWave ? bn : nz;
-		double s = r < nrows ? lds_strided_sum(p, a + sub, b, L) : 0.0;
-		s = group_sum(s, L);
-		if (r < nrows && sub == 0)
-			y[r0 + r] = s;
-	}
-	wave_lds_sync();
+	block_rows_reduce(p, st.ra, nrows, nz, lanes_cfg, y + r0);
 }
 
 // x[c] for a SELL entry. With XW the workgroup has copied x[lo, lo + W) to

@@ -124,6 +124,27 @@ class DeviceCSR:
         return hip.csr_to_format_device(self.rp, self.col, self.val, self.m, self.n, self.nnz, kind=kind, **kw)
 
 
+def batch_variant(i, min_block_nnz=256):
+    """The variant a matrix runs as inside a persistent batched launch.
+
+    The selector and the autotuner rank variants by standalone launches,
+    where a small matrix gains from small CSR row blocks (more waves in
+    flight). In a batch the other members supply that parallelism, and a
+    small block only adds per-unit overhead (descriptor loads, a reduction)
+    to a wave's sequential stream, so a CSR choice with fewer than
+    `min_block_nnz` nonzeros per block is raised to that block size with the
+    same lanes-per-row setting. Other choices are kept. 0 keeps every choice.
+    """
+    kind, kw = VARIANTS[i]
+    if kind != "csr_vec" or not min_block_nnz or kw.get("block_nnz", 256) >= min_block_nnz:
+        return i
+    want = dict(kw, block_nnz=min_block_nnz)
+    for j, (k2, kw2) in enumerate(VARIANTS):
+        if k2 == kind and kw2 == want:
+            return j
+    return i
+
+
 def time_variant(dm, i, warmup=2, iters=8):
     f = dm.convert(i)
     try:
