@@ -39,23 +39,36 @@ constexpr int kTailItemUnits = 4;  // default units per work item of the dynamic
 constexpr int kTailRounds = 4;     // the tail is the last kTailRounds * W units
 constexpr int kCounterStride = 64; // ints between queue counters: one 256-byte line each
 
-// A member's device arrays, as pointer values (common.hpp: global_ptr).
+// A member's device arrays, as pointer values (common.hpp: global_ptr). The
+// fields every unit reads come first, so that a wave fetches them with one
+// scalar load.
 struct BatchMat {
-	int kind, lanes;
+	uint64_t x, y;      // double
+	uint64_t col, val;  // CSR: col int32, val double; SELL*: slice columns int32, slice values double
+	uint64_t rp;        // CSR: row_ptr int32; SELL*: slice_ptr int64
+	uint64_t sw;        // SELL*: slice_w int32 (0 for CSR)
+	uint64_t order;     // SELL-C-sigma: row_order int32 (0 otherwise)
+	uint64_t partials;  // CSR with split rows: segment partial sums, double (0 otherwise)
 	long m;
-	uint64_t rp, col, blocks, val;         // int32, int32, int32, double (CSR_VEC)
-	uint64_t sptr, sw, scol, order, sval;  // int64, int32, int32, int32 (0 for SELL), double (SELL*)
-	uint64_t x, y;                         // double
-	// CSR rows longer than split_nnz: segments (row, j0, j1) summed into
-	// partials, added per row by batch_split_finalize
-	uint64_t seg_desc, seg_ptr, split_rows, partials; // int32 x 3, double (0 without split rows)
-	int split_nnz;
+	int lanes, split_nnz;
+	// read by batch_split_finalize only
+	uint64_t seg_ptr, split_rows; // int32
 };
 
-// one unit of the global list: member `mat`, its unit u >= 0 (row block or
-// slice) or its split-row segment -u - 1
-struct UnitRef {
-	int mat, u;
+enum : int { kUnitCsr = 0, kUnitSell = 1, kUnitSeg = 2 };
+constexpr int kKindShift = 28; // UnitRec::mk = member index | kind << 28
+constexpr int kMatMask = (1 << kKindShift) - 1;
+
+// One unit of the global list, precomputed on the device at finalize: a CSR
+// row block (rows [r0, r1), nonzeros [j0, j1)), a SELL slice u, or split-row
+// segment u (nonzeros [j0, j1)). With the bounds in the record a wave
+// reaches its stream after two dependent scalar loads (the record, then the
+// member's pointers) instead of four (record, member, block table, row
+// pointers).
+struct alignas(8) UnitRec {
+	int mk, u;
+	int r0, r1;
+	int j0, j1;
 };
 
 struct WorkItem {
@@ -74,27 +87,49 @@ __global__ void batch_reset(int *counter)
 		counter[threadIdx.x * kCounterStride] = 0;
 }
 
-template <int BNNZ>
-__device__ __forceinline__ void batch_unit(const BatchMat &M, int u, double *prod)
+__global__ void batch_fill_csr(const int32_t *__restrict__ blocks, const int32_t *__restrict__ rp, int nb, int mk,
+		UnitRec *__restrict__ out)
 {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nb)
+		return;
+	const int r0 = blocks[i], r1 = blocks[i + 1];
+	out[i] = UnitRec{mk, i, r0, r1, rp[r0], rp[r1]};
+}
+
+__global__ void batch_fill_seg(const int32_t *__restrict__ desc, int ns, int mk, UnitRec *__restrict__ out)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < ns)
+		out[i] = UnitRec{mk, i, 0, 0, desc[3 * i + 1], desc[3 * i + 2]};
+}
+
+__global__ void batch_fill_sell(int ns, int mk, UnitRec *__restrict__ out)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < ns)
+		out[i] = UnitRec{mk, i, 0, 0, 0, 0};
+}
+
+template <int BNNZ>
+__device__ __forceinline__ void batch_unit(const BatchMat *__restrict__ mats, const UnitRec &r, double *prod)
+{
+	const int kind = r.mk >> kKindShift;
+	const BatchMat &M = mats[r.mk & kMatMask];
 	const double *x = global_ptr<const double>(M.x);
 	double *y = global_ptr<double>(M.y);
-	if (u < 0) {
-		const int s = -u - 1;
-		const int32_t *desc = const_ptr<int32_t>(M.seg_desc);
-		const double sum = wave_row_sum(global_ptr<const int32_t>(M.col), global_ptr<const double>(M.val), x,
-				desc[3 * s + 1], desc[3 * s + 2]);
+	const int32_t *col = global_ptr<const int32_t>(M.col);
+	const double *val = global_ptr<const double>(M.val);
+	if (kind == kUnitCsr) {
+		csr_block_at<BNNZ>(global_ptr<const int32_t>(M.rp), col, val, x, y, r.r0, r.r1, r.j0, r.j1, M.split_nnz,
+				M.partials != 0, M.lanes, prod);
+	} else if (kind == kUnitSeg) {
+		const double sum = wave_row_sum(col, val, x, r.j0, r.j1);
 		if (lane_id() == 0)
-			global_ptr<double>(M.partials)[s] = sum;
-	} else if (M.kind == SPMVHIP_CSR_VEC) {
-		const int32_t *blocks = const_ptr<int32_t>(M.blocks);
-		csr_block<BNNZ>(const_ptr<int32_t>(M.rp), global_ptr<const int32_t>(M.col),
-				global_ptr<const double>(M.val), x, y, blocks[u], blocks[u + 1], M.split_nnz, M.partials != 0,
-				M.lanes, prod);
+			global_ptr<double>(M.partials)[r.u] = sum;
 	} else {
-		sell_slice<4>(const_ptr<int64_t>(M.sptr), const_ptr<int32_t>(M.sw),
-				global_ptr<const int32_t>(M.scol), global_ptr<const double>(M.sval),
-				M.order ? global_ptr<const int32_t>(M.order) : nullptr, x, y, M.m, u, M.lanes);
+		sell_slice<4>(const_ptr<int64_t>(M.rp), const_ptr<int32_t>(M.sw), col, val,
+				M.order ? global_ptr<const int32_t>(M.order) : nullptr, x, y, M.m, r.u, M.lanes);
 	}
 }
 
@@ -106,18 +141,18 @@ __device__ __forceinline__ void batch_unit(const BatchMat &M, int u, double *pro
 // compared on the scalar unit. A 64-bit signed compare has no scalar form, and
 // the VGPRs it borrowed at the loop latch made the compiler wait there for
 // every outstanding memory operation of the wave, the y stores included.
+// The kernel holds all unit kinds' code; it is kept to 64 VGPRs so that 8
+// waves per SIMD stay resident.
 template <int BNNZ>
-__global__ void __launch_bounds__(kBlock) batch_kernel(const BatchMat *__restrict__ mats,
-		const UnitRef *__restrict__ units, int nstatic, const WorkItem *__restrict__ items,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) batch_kernel(const BatchMat *__restrict__ mats,
+		const UnitRec *__restrict__ units, int nstatic, const WorkItem *__restrict__ items,
 		const int *__restrict__ qstart, int nqueues, int *__restrict__ counter)
 {
 	__shared__ double prod[kWavesPerBlock][2 * BNNZ];
 	const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
 	const int W = (int) gridDim.x * kWavesPerBlock;
-	for (int g = (int) blockIdx.x * kWavesPerBlock + wv; g < nstatic; g += W) {
-		const UnitRef r = units[g];
-		batch_unit<BNNZ>(mats[r.mat], r.u, prod[wv]);
-	}
+	for (int g = (int) blockIdx.x * kWavesPerBlock + wv; g < nstatic; g += W)
+		batch_unit<BNNZ>(mats, units[g], prod[wv]);
 	const int q = (int) (blockIdx.x % nqueues);
 	const int q0 = qstart[q], q1 = qstart[q + 1];
 	if (q0 == q1)
@@ -130,10 +165,8 @@ __global__ void __launch_bounds__(kBlock) batch_kernel(const BatchMat *__restric
 		if (q0 + t >= q1)
 			break; // queue drained: wave-uniform exit, every wave reaches it
 		const WorkItem wi = items[q0 + t];
-		for (int g = wi.g0; g < wi.g1; g++) {
-			const UnitRef r = units[g];
-			batch_unit<BNNZ>(mats[r.mat], r.u, prod[wv]);
-		}
+		for (int g = wi.g0; g < wi.g1; g++)
+			batch_unit<BNNZ>(mats, units[g], prod[wv]);
 	}
 }
 
@@ -154,6 +187,14 @@ __global__ void batch_split_finalize(const BatchMat *__restrict__ mats, const Sp
 	global_ptr<double>(M.y)[global_ptr<const int32_t>(M.split_rows)[i]] = s;
 }
 
+// A member's share of the unit list: `count` records from `off`, filled at
+// finalize from the member's device tables.
+struct UnitSpan {
+	int mk;
+	long off, count;
+	const int32_t *blocks, *rp, *seg_desc; // CSR blocks / segments (device)
+};
+
 } // namespace spmvhip
 
 struct spmvhip_batch {
@@ -162,7 +203,8 @@ struct spmvhip_batch {
 	int queues = 8;                     // tail queues: 8 = one per XCD, 1 = shared
 	std::vector<spmvhip_format *> fmts;
 	std::vector<spmvhip::BatchMat> host_mats;
-	std::vector<spmvhip::UnitRef> host_units;
+	std::vector<spmvhip::UnitSpan> spans;
+	long nunits = 0;
 	std::vector<spmvhip::WorkItem> host_items;
 	std::vector<spmvhip::SplitRef> host_splits;
 	std::vector<int> host_qstart;
@@ -184,12 +226,12 @@ int batch_launch(spmvhip_batch *b, hipStream_t s)
 {
 	batch_reset<<<1, kWave, 0, s>>>(b->counter.as<int>());
 	SPMVHIP_CHECK(hipGetLastError());
-	if (!b->host_units.empty()) {
+	if (b->nunits > 0) {
 		if (b->bnnz > 256)
-			batch_kernel<kBatchBnnzMax><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->units.as<UnitRef>(),
+			batch_kernel<kBatchBnnzMax><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->units.as<UnitRec>(),
 					b->nstatic, b->items.as<WorkItem>(), b->qstart.as<int>(), b->queues, b->counter.as<int>());
 		else
-			batch_kernel<256><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->units.as<UnitRef>(),
+			batch_kernel<256><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->units.as<UnitRec>(),
 					b->nstatic, b->items.as<WorkItem>(), b->qstart.as<int>(), b->queues, b->counter.as<int>());
 		SPMVHIP_CHECK(hipGetLastError());
 	}
@@ -245,24 +287,23 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 	if (b->finalized)
 		return set_error(SPMVHIP_EINVAL, "batch already finalized");
 	BatchMat M{};
-	M.kind = f->kind;
 	M.m = f->m;
 	M.x = ptr_bits(d_x);
 	M.y = ptr_bits(d_y);
 	long units, nsegs = 0;
+	int kind;
 	if (f->kind == SPMVHIP_CSR_VEC) {
 		if (f->block_nnz > kBatchBnnzMax)
 			return set_error(SPMVHIP_EINVAL, "batch: csr_vec block_nnz %d > %d", f->block_nnz, kBatchBnnzMax);
 		if (f->block_nnz > b->bnnz)
 			b->bnnz = kBatchBnnzMax;
+		kind = kUnitCsr;
 		M.rp = ptr_bits(f->row_ptr.p);
 		M.col = ptr_bits(f->col.p);
 		M.val = ptr_bits(f->val.p);
-		M.blocks = ptr_bits(f->blocks.p);
 		M.lanes = f->lanes;
 		M.split_nnz = f->split_nnz;
 		if (f->nsegs > 0) {
-			M.seg_desc = ptr_bits(f->seg_desc.p);
 			M.seg_ptr = ptr_bits(f->seg_ptr.p);
 			M.split_rows = ptr_bits(f->split_rows.p);
 			M.partials = ptr_bits(f->partials.p);
@@ -270,10 +311,11 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 		}
 		units = f->nblocks;
 	} else if (f->kind == SPMVHIP_SELL || f->kind == SPMVHIP_SELL_C_S) {
-		M.sptr = ptr_bits(f->slice_ptr.p);
+		kind = kUnitSell;
+		M.rp = ptr_bits(f->slice_ptr.p);
 		M.sw = ptr_bits(f->slice_w.p);
-		M.scol = ptr_bits(f->s_cols.p);
-		M.sval = ptr_bits(f->s_vals.p);
+		M.col = ptr_bits(f->s_cols.p);
+		M.val = ptr_bits(f->s_vals.p);
 		M.order = f->kind == SPMVHIP_SELL_C_S ? ptr_bits(f->row_order.p) : 0;
 		M.lanes = f->lanes; // lanes per row T (>= 1) of the slice layout
 		units = f->nslices;
@@ -281,18 +323,25 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 		return set_error(SPMVHIP_EINVAL, "batch: format %s is not batchable (csr_vec, sell, sell_c_s are)",
 				f->name);
 	}
-	if (units > INT32_MAX || nsegs > INT32_MAX || b->host_mats.size() >= (size_t) INT32_MAX)
-		return set_error(SPMVHIP_EINVAL, "batch: too many units");
+	if (units > INT32_MAX || nsegs > INT32_MAX || b->host_mats.size() >= (size_t) kMatMask)
+		return set_error(SPMVHIP_EINVAL, "batch: too many units or members");
 	const int mi = (int) b->host_mats.size();
 	b->host_mats.push_back(M);
 	b->fmts.push_back(const_cast<spmvhip_format *>(f));
 	// the member's segments (its longest work) first, then its units
-	for (long s = 0; s < nsegs; s++)
-		b->host_units.push_back({mi, (int) (-s - 1)});
-	for (long i = 0; i < f->nsplit && nsegs > 0; i++)
-		b->host_splits.push_back({mi, (int) i});
-	for (long u = 0; u < units; u++)
-		b->host_units.push_back({mi, (int) u});
+	if (nsegs > 0) {
+		b->spans.push_back({mi | kUnitSeg << kKindShift, b->nunits, nsegs, nullptr, nullptr,
+				f->seg_desc.as<int32_t>()});
+		b->nunits += nsegs;
+		for (long i = 0; i < f->nsplit; i++)
+			b->host_splits.push_back({mi, (int) i});
+	}
+	if (kind == kUnitCsr)
+		b->spans.push_back({mi | kind << kKindShift, b->nunits, units, f->blocks.as<int32_t>(),
+				f->row_ptr.as<int32_t>(), nullptr});
+	else
+		b->spans.push_back({mi | kind << kKindShift, b->nunits, units, nullptr, nullptr, nullptr});
+	b->nunits += units;
 	b->total_nnz += f->nnz;
 	return SPMVHIP_OK;
 }
@@ -317,7 +366,7 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 	b->grid = (int) g;
 	// the static sweep takes all but the last kTailRounds * W units; the tail
 	// becomes items of `tile` units dealt round-robin to the queues
-	const long N = (long) b->host_units.size(), W = g * kWavesPerBlock;
+	const long N = b->nunits, W = g * kWavesPerBlock;
 	if (N + W > INT32_MAX)
 		return set_error(SPMVHIP_EINVAL, "batch: %ld work units (the kernel indexes them with 32 bits)", N);
 	b->nstatic = (int) std::max<long>(0, N - kTailRounds * W);
@@ -335,14 +384,28 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 
 	const size_t nm = b->host_mats.size();
 	SPMVHIP_TRY(b->mats.alloc(std::max<size_t>(1, nm) * sizeof(BatchMat)));
-	SPMVHIP_TRY(b->units.alloc(std::max<size_t>(1, b->host_units.size()) * sizeof(UnitRef)));
+	SPMVHIP_TRY(b->units.alloc(std::max<long>(1, N) * sizeof(UnitRec)));
 	SPMVHIP_TRY(b->items.alloc(std::max<size_t>(1, b->host_items.size()) * sizeof(WorkItem)));
 	SPMVHIP_TRY(b->qstart.alloc((kXcds + 1) * sizeof(int)));
 	SPMVHIP_TRY(b->counter.alloc(kXcds * kCounterStride * sizeof(int)));
 	if (nm)
 		SPMVHIP_CHECK(hipMemcpy(b->mats.p, b->host_mats.data(), nm * sizeof(BatchMat), hipMemcpyHostToDevice));
-	if (N)
-		SPMVHIP_CHECK(hipMemcpy(b->units.p, b->host_units.data(), N * sizeof(UnitRef), hipMemcpyHostToDevice));
+	// the unit records, from each member's device tables
+	for (const UnitSpan &sp : b->spans) {
+		if (sp.count == 0)
+			continue;
+		UnitRec *out = b->units.as<UnitRec>() + sp.off;
+		const unsigned grid = grid_for(sp.count, kBlock);
+		const int n = (int) sp.count;
+		if (sp.blocks)
+			batch_fill_csr<<<grid, kBlock, 0, b->stream>>>(sp.blocks, sp.rp, n, sp.mk, out);
+		else if (sp.seg_desc)
+			batch_fill_seg<<<grid, kBlock, 0, b->stream>>>(sp.seg_desc, n, sp.mk, out);
+		else
+			batch_fill_sell<<<grid, kBlock, 0, b->stream>>>(n, sp.mk, out);
+		SPMVHIP_CHECK(hipGetLastError());
+	}
+	SPMVHIP_CHECK(hipStreamSynchronize(b->stream));
 	if (!b->host_items.empty())
 		SPMVHIP_CHECK(hipMemcpy(b->items.p, b->host_items.data(), b->host_items.size() * sizeof(WorkItem),
 				hipMemcpyHostToDevice));
@@ -433,7 +496,7 @@ int spmvhip_batch_info(const spmvhip_batch *b, long *nmats, long *nitems, int *g
 	if (nmats)
 		*nmats = (long) b->host_mats.size();
 	if (nitems)
-		*nitems = (long) b->host_units.size();
+		*nitems = b->nunits;
 	if (grid)
 		*grid = b->grid;
 	if (total_nnz)

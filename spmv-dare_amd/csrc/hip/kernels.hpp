@@ -123,16 +123,20 @@ __device__ __forceinline__ void block_rows_reduce(const double *p, int ra, int n
 // requested per step, lanes past the block's end re-read its last entry and
 // store a zero product, so a block of <= 256 nonzeros costs one index round
 // trip and one gather round trip.
+//
+// csr_block_at takes the block's nonzero range [j0, j1) = [rp[r0], rp[r1])
+// already known (a precomputed work-unit record); csr_block loads it.
 template <int BNNZ_MAX>
-__device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+__device__ __forceinline__ void csr_block_at(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
 		const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y, int r0, int r1,
-		int split_nnz, bool skip_split, int lanes_cfg, double *p)
+		int j0, int j1, int split_nnz, bool skip_split, int lanes_cfg, double *p)
 {
 	static_assert(BNNZ_MAX % 128 == 0, "the stream phase writes p in steps of 256 entries");
 	const int lane = lane_id();
 	r0 = __builtin_amdgcn_readfirstlane(r0);
 	r1 = __builtin_amdgcn_readfirstlane(r1);
-	const int j0 = __builtin_amdgcn_readfirstlane(rp[r0]), j1 = __builtin_amdgcn_readfirstlane(rp[r1]);
+	j0 = __builtin_amdgcn_readfirstlane(j0);
+	j1 = __builtin_amdgcn_readfirstlane(j1);
 	const int nrows = r1 - r0, nz = j1 - j0;
 	if (nrows == 1 && nz > BNNZ_MAX) {
 		if (skip_split && nz > split_nnz)
@@ -170,6 +174,16 @@ __device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const 
 	}
 	wave_lds_sync();
 	block_rows_reduce(p, ra, nrows, nz, lanes_cfg, y + r0);
+}
+
+template <int BNNZ_MAX>
+__device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+		const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y, int r0, int r1,
+		int split_nnz, bool skip_split, int lanes_cfg, double *p)
+{
+	r0 = __builtin_amdgcn_readfirstlane(r0);
+	r1 = __builtin_amdgcn_readfirstlane(r1);
+	csr_block_at<BNNZ_MAX>(rp, col, val, x, y, r0, r1, rp[r0], rp[r1], split_nnz, skip_split, lanes_cfg, p);
 }
 
 // The same row block split in three stages, so that a persistent wave can
