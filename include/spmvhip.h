@@ -76,7 +76,7 @@ typedef struct spmvhip_params {
 	                         256 rows in LDS when it spans <= this many doubles (1 = 2048;
 	                         at most 16384); 0 = off */
 	int unroll;           /* SELL*: inner unroll (1, 2, 4, 8; default 8); DIV_RF: values per lane
-	                         and decode step (4, or 8 when >= 8; default 4) */
+	                         and decode step (4, 8 when >= 8, 16 when >= 16; default 4) */
 	int div_threads;      /* DIV_RF: encoder partitions T (default 1) */
 	int div_lanes;        /* DIV_RF: lanes L of the value streams (default 64) */
 	long div_packet_vals; /* DIV_RF: P, max values per packet (default 16384) */

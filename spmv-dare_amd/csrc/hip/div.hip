@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(kBlock) div_spmv_kernel(const uint8_t *__restr
 // reads of the decode are LDS reads; only the x gathers go to global memory.
 // The arithmetic is div_spmv_kernel's.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-constexpr int kDivStageRegs = 4; // 16-byte loads per lane in flight per round of the packet copy
+constexpr int kDivStageRegs = 12; // 16-byte loads per lane in flight per round of the packet copy (12 KiB)
 
 __device__ __forceinline__ uint64_t lds_le(const uint8_t *base, long pos, int nbytes)
 {
@@ -356,10 +356,15 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 	const long g = blockIdx.x;
 	if (g >= ngroups)
 		return;
-	for (long r = grp_row[g] + lane; r < grp_row[g + 1]; r += kWave)
-		y[r] = 0.0;
 	const int p_end = grp_pk[g + 1];
 	int p = grp_pk[g];
+	// a group of one packet (the common case) stores its rows once at the
+	// end; a longer group zeroes them first and adds each packet's rows
+	const bool single = p_end - p == 1;
+	const long gr0 = grp_row[g], gr1 = grp_row[g + 1];
+	if (!single)
+		for (long r = gr0 + lane; r < gr1; r += kWave)
+			y[r] = 0.0;
 	for (; p < p_end; p++) {
 		// copy the packet to LDS in rounds of kDivStageRegs coalesced 16-byte
 		// loads per lane, all of a round in flight before its LDS writes
@@ -453,8 +458,16 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 			cs += n * rf;
 		}
 		wave_fence();
-		for (uint32_t r = lane; r < nrows; r += kWave)
-			y[row_min + r] += yw[r];
+		if (single) {
+			// the group's rows outside the packet are empty rows
+			for (long r = gr0 + lane; r < gr1; r += kWave) {
+				const long i = r - (long) row_min;
+				y[r] = i >= 0 && i < (long) nrows ? yw[i] : 0.0;
+			}
+		} else {
+			for (uint32_t r = lane; r < nrows; r += kWave)
+				y[row_min + r] += yw[r];
+		}
 		wave_fence(); // the packet buffer and accumulators are reused
 	}
 }
@@ -537,7 +550,11 @@ int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 	}
 	if (f->div_buf_bytes > 0) {
 		const size_t lds = (size_t) f->div_buf_bytes + 256 * sizeof(double);
-		if (f->p.unroll >= 8)
+		if (f->p.unroll >= 16)
+			div_spmv_lds_kernel<16><<<(unsigned) f->div_ngroups, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
+					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
+					f->div_ngroups, f->div_lanes, f->div_buf_bytes, x, y);
+		else if (f->p.unroll >= 8)
 			div_spmv_lds_kernel<8><<<(unsigned) f->div_ngroups, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
 					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
 					f->div_ngroups, f->div_lanes, f->div_buf_bytes, x, y);

@@ -23,12 +23,14 @@ def device():
 
 @pytest.mark.parametrize("name", NAMES)
 @pytest.mark.parametrize("T,L,P", CODECS)
-def test_div_device_decode_and_spmv(name, T, L, P):
+@pytest.mark.parametrize("unroll", [4, 8, 16])
+def test_div_device_decode_and_spmv(name, T, L, P, unroll):
+    """unroll picks the decoder instance: 4, 8 or 16 values per lane and step."""
     rp, ci, va, n = matrix(name)
     m = rp.size - 1
-    print("case", name, T, L, P, flush=True)
+    print("case", name, T, L, P, unroll, flush=True)
     f = hip.csr_to_format(rp, ci, va, m, n, ci.size, kind="div_rf", div_threads=T, div_lanes=L,
-                          div_packet_vals=P)
+                          div_packet_vals=P, unroll=unroll)
     if ci.size:
         b, off, grp = hip.div_encode(rp, ci, va, n, T=T, L=L, P=P)
         r, c, v = pyoracle.div_decode(b, L, ci.size)
