@@ -191,6 +191,9 @@ int spmvhip_batch_create(int tile_units, spmvhip_batch **out);
 /* Tail queues (before finalize): 8 (default) = one per XCD, items dealt
  * round-robin; 1 = one shared queue. */
 int spmvhip_batch_set_queues(spmvhip_batch *b, int nqueues);
+/* Persistent grid (before finalize): workgroups of 4 wavefronts per CU, 1..8,
+ * at most what the kernel's resources allow; 0 (default) = 4. */
+int spmvhip_batch_set_grid(spmvhip_batch *b, int workgroups_per_cu);
 int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d_x, double *d_y);
 int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph);
 int spmvhip_batch_run(spmvhip_batch *b);          /* asynchronous on the batch's stream */

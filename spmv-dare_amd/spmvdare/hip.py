@@ -71,6 +71,7 @@ _sig = {
     "spmvhip_destroy": ([_vp], None),
     "spmvhip_batch_create": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "spmvhip_batch_set_queues": ([_vp, ctypes.c_int], ctypes.c_int),
+    "spmvhip_batch_set_grid": ([_vp, ctypes.c_int], ctypes.c_int),
     "spmvhip_batch_add": ([_vp, _vp, _vp, _vp], ctypes.c_int),
     "spmvhip_batch_finalize": ([_vp, ctypes.c_int], ctypes.c_int),
     "spmvhip_batch_run": ([_vp], ctypes.c_int),
@@ -287,6 +288,10 @@ class Batch:
     def set_queues(self, nqueues):
         """8 (default): one work queue per XCD; 1: one shared queue."""
         _check(_lib.spmvhip_batch_set_queues(self._h, nqueues), "batch_set_queues")
+
+    def set_grid(self, workgroups_per_cu):
+        """Resident workgroups per CU of the persistent launch (0 = default)."""
+        _check(_lib.spmvhip_batch_set_grid(self._h, workgroups_per_cu), "batch_set_grid")
 
     def add(self, fmt, dx, dy):
         _check(_lib.spmvhip_batch_add(self._h, fmt._h, dx.ptr, dy.ptr), "batch_add")

@@ -129,6 +129,28 @@ def test_batch_split_rows(members, use_graph, queues):
     b.destroy()
 
 
+@pytest.mark.parametrize("wgs", [1, 3, 8])
+def test_batch_grid_sizes(members, wgs):
+    """Any resident grid (workgroups per CU) computes the same y: the static
+    sweep and the ticketed tail cover every unit exactly once."""
+    b = hip.Batch(2)
+    b.set_grid(wgs)
+    for mb in members:
+        b.add(mb["fmt"], mb["dx"], mb["dy"])
+    b.finalize(use_graph=True)
+    print("batch", b.info(), flush=True)
+    for mb in members:
+        mb["dy"].upload(np.full(mb["m"], np.nan))
+    b.run()
+    b.synchronize()
+    check(members)
+    b.destroy()
+    bad = hip.Batch()
+    with pytest.raises(hip.SpmvHipError):
+        bad.set_grid(9)
+    bad.destroy()
+
+
 def test_batch_rejects_unbatchable(members):
     a = gen.generate(gen.GenParams.parse(GEN["test_txt"]["line"]), stats=False)
     b = hip.Batch()
