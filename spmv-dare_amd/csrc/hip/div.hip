@@ -255,13 +255,13 @@ __global__ void __launch_bounds__(kBlock) div_spmv_kernel(const uint8_t *__restr
 
 // The decoder with every packet staged in LDS (one wavefront per workgroup,
 // dynamic LDS = the largest packet rounded up + 16 bytes of padding, plus
-// the packet's 256 row accumulators). A packet is copied to LDS with
-// coalesced 16-byte loads, all requested before the first LDS write; the
-// other resident waves hide the latency. All stream, length and coordinate
-// reads of the decode are LDS reads; only the x gathers go to global memory.
-// The arithmetic is div_spmv_kernel's.
+// row accumulators for the largest packet's rows, so that as many
+// workgroups fit per CU as the packets allow). A packet is copied to LDS by
+// direct global-to-LDS 16-byte loads, all requested at once; the other
+// resident waves hide the latency. All stream, length and coordinate reads
+// of the decode are LDS reads; only the x gathers go to global memory. The
+// arithmetic is div_spmv_kernel's.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-constexpr int kDivStageRegs = 12; // 16-byte loads per lane in flight per round of the packet copy (12 KiB)
 
 __device__ __forceinline__ uint64_t lds_le(const uint8_t *base, long pos, int nbytes)
 {
@@ -300,46 +300,57 @@ __device__ __forceinline__ uint32_t lds_le_b32x(const uint8_t *base, int pos, ui
 // inside the packet), gathers x[col_min] (a valid column) and decodes a zero
 // delta; only its LDS accumulation is skipped. C4: the coordinates fit 4
 // bytes and are read with one byte-align.
+//
+// A step of K values runs in two passes: the first decodes the K columns
+// and requests all K gathers of x (one memory round trip per step), the
+// second re-reads each coordinate and length byte from LDS (cheap) to decode
+// the value and accumulate. Only the K gathered values stay in registers
+// across the passes.
+template <int K, bool C4>
+__device__ __forceinline__ uint32_t div_coord(const uint8_t *pk, int at, int cb, uint32_t cmask, int row_bits,
+		uint32_t &row)
+{
+	const uint32_t rmask = (1u << row_bits) - 1; // row_bits <= 8 (at most 255 rows per packet)
+	if (C4) {
+		const uint32_t c = lds_le_b32x(pk, at, cmask);
+		row = c & rmask;
+		return c >> row_bits;
+	}
+	const uint64_t c = lds_le_b32(pk, at, cb);
+	row = (uint32_t) c & rmask;
+	return (uint32_t) (c >> row_bits);
+}
+
 template <int K, bool C4>
 __device__ __forceinline__ void div_rf1_group(const uint8_t *pk, int coords_at, int lens_at, int lens0, int crd0_at,
 		int stream_at, int steps, int cnt, int L, int cb, uint32_t col_min, int row_bits,
 		const double *__restrict__ x, double *yw, int &cursor)
 {
 	const uint32_t cmask = cb >= 4 ? 0xffffffffu : (1u << (8 * cb)) - 1;
-	const uint32_t rmask = (1u << row_bits) - 1; // row_bits <= 8 (at most 255 rows per packet)
 	uint64_t val = 0;
 	for (int p0 = 0; p0 < steps; p0 += K) {
-		uint32_t row[K];
 		double xv[K];
-		int len[K], tz[K];
-		bool ok[K];
 #pragma unroll
 		for (int k = 0; k < K; k++) {
-			ok[k] = p0 + k < cnt;
-			const int at = ok[k] ? crd0_at + (p0 + k) * L * cb : coords_at;
-			uint32_t col;
-			if (C4) {
-				const uint32_t c = lds_le_b32x(pk, at, cmask);
-				row[k] = c & rmask;
-				col = c >> row_bits;
-			} else {
-				const uint64_t c = lds_le_b32(pk, at, cb);
-				row[k] = (uint32_t) c & rmask;
-				col = (uint32_t) (c >> row_bits);
-			}
-			xv[k] = x[col_min + (ok[k] ? col : 0u)];
-			const int lb = pk[ok[k] ? lens0 + (p0 + k) * L : lens_at];
-			len[k] = ok[k] ? lb & 15 : 0;
-			tz[k] = (lb >> 4) << 2;
+			const bool ok = p0 + k < cnt;
+			uint32_t row;
+			const uint32_t col = div_coord<K, C4>(pk, ok ? crd0_at + (p0 + k) * L * cb : coords_at, cb, cmask,
+					row_bits, row);
+			xv[k] = x[col_min + (ok ? col : 0u)];
 		}
 #pragma unroll
 		for (int k = 0; k < K; k++) {
-			val += lds_le_b32(pk, stream_at + cursor, len[k]) << tz[k];
-			cursor += len[k];
+			const bool ok = p0 + k < cnt;
+			uint32_t row;
+			(void) div_coord<K, C4>(pk, ok ? crd0_at + (p0 + k) * L * cb : coords_at, cb, cmask, row_bits, row);
+			const int lb = pk[ok ? lens0 + (p0 + k) * L : lens_at];
+			const int len = ok ? lb & 15 : 0, tz = (lb >> 4) << 2;
+			val += lds_le_b32(pk, stream_at + cursor, len) << tz;
+			cursor += len;
 			double d;
 			__builtin_memcpy(&d, &val, 8);
-			if (ok[k]) // (an unconditional add of 0.0 would pile every idle lane onto one LDS address)
-				atomicAdd(&yw[row[k]], d * xv[k]);
+			if (ok) // (an unconditional add of 0.0 would pile every idle lane onto one LDS address)
+				atomicAdd(&yw[row], d * xv[k]);
 		}
 	}
 }
@@ -351,7 +362,7 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 {
 	extern __shared__ u32x4 smem[];
 	uint8_t *pk = (uint8_t *) smem;                   // buf_bytes (a multiple of 16, padding included)
-	double *yw = (double *) (pk + buf_bytes);          // 256 row accumulators
+	double *yw = (double *) (pk + buf_bytes);          // row accumulators (the most rows of any packet)
 	const int lane = lane_id();
 	const long g = blockIdx.x;
 	if (g >= ngroups)
@@ -366,25 +377,18 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 		for (long r = gr0 + lane; r < gr1; r += kWave)
 			y[r] = 0.0;
 	for (; p < p_end; p++) {
-		// copy the packet to LDS in rounds of kDivStageRegs coalesced 16-byte
-		// loads per lane, all of a round in flight before its LDS writes
+		// copy the packet to LDS with direct global-to-LDS 16-byte loads (lane i
+		// of a round writes bytes [16 i, 16 i + 16) of the round's 1 KiB; lanes
+		// past the packet are masked off, so no write leaves the packet
+		// buffer), all rounds in flight at once, no VGPR staging
 		const u32x4 *src = (const u32x4 *) (bytes + pk_off[p]);
 		const int n16 = (int) ((pk_off[p + 1] - pk_off[p] + 15) >> 4);
-		for (int j0 = 0; j0 < n16; j0 += kDivStageRegs * kWave) {
-			u32x4 stage[kDivStageRegs];
-#pragma unroll
-			for (int i = 0; i < kDivStageRegs; i++) {
-				const int j = j0 + i * kWave + lane;
-				if (j < n16)
-					stage[i] = __builtin_nontemporal_load(src + j);
-			}
-#pragma unroll
-			for (int i = 0; i < kDivStageRegs; i++) {
-				const int j = j0 + i * kWave + lane;
-				if (j < n16)
-					smem[j] = stage[i];
-			}
+		for (int j0 = 0; j0 < n16; j0 += kWave) {
+			if (j0 + lane < n16)
+				__builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *) (src + j0 + lane),
+						(__attribute__((address_space(3))) void *) (smem + j0), 16, 0, 0);
 		}
+		__builtin_amdgcn_s_waitcnt(0); // the copies have landed in LDS
 		if (lane == 0) // zero padding after the packet: the decode reads whole words
 			smem[n16] = u32x4{0, 0, 0, 0};
 		wave_fence();
@@ -517,8 +521,16 @@ int div_convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col, co
 	for (long p = 0; p < np; p++)
 		max_pk = std::max<int64_t>(max_pk, doff[p + 1] - doff[p]);
 	// the staged decoder keeps a packet (16-byte rounded, + 16 of zero padding)
-	// and 256 row accumulators in LDS: used while that fits 64 KiB
-	f->div_buf_bytes = max_pk + 16 + 2048 <= kDivLdsMax ? (int) (max_pk + 16) : 0;
+	// and one accumulator per row of the packet in LDS: used while that fits
+	// 64 KiB. Header word 2 is the packet's row count (at most 255).
+	uint32_t max_rows = 1;
+	for (long p = 0; p < np; p++) {
+		uint32_t nr;
+		memcpy(&nr, &st.bytes[st.pk_off[p] + 8], sizeof(nr));
+		max_rows = std::max(max_rows, nr);
+	}
+	f->div_acc_bytes = (int) ((max_rows * sizeof(double) + 15) / 16 * 16);
+	f->div_buf_bytes = max_pk + 16 + f->div_acc_bytes <= kDivLdsMax ? (int) (max_pk + 16) : 0;
 	SPMVHIP_TRY(f->div_stream.alloc(dev.size()));
 	SPMVHIP_TRY(f->div_pk_off.alloc((np + 1) * sizeof(int64_t)));
 	SPMVHIP_TRY(f->div_pk_nnz.alloc((np + 1) * sizeof(int64_t)));
@@ -549,7 +561,7 @@ int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 		return SPMVHIP_OK;
 	}
 	if (f->div_buf_bytes > 0) {
-		const size_t lds = (size_t) f->div_buf_bytes + 256 * sizeof(double);
+		const size_t lds = (size_t) f->div_buf_bytes + (size_t) f->div_acc_bytes;
 		if (f->p.unroll >= 16)
 			div_spmv_lds_kernel<16><<<(unsigned) f->div_ngroups, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
 					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
