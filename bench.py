@@ -395,12 +395,12 @@ def main():
     ap.add_argument("--batched", action="store_true", help="batched workload instead of one matrix")
     ap.add_argument("--count", type=int, default=2048, help="matrices of the batched workload")
     ap.add_argument("--tile", type=int, default=0,
-                    help="batched launch: units per work item of the ticketed tail (0 = the library default)")
+                    help="batched launch: consecutive units per ticket (0 = the library default)")
     ap.add_argument("--no-selector", action="store_true")
     ap.add_argument("--batch-bnnz", type=int, default=256,
                     help="batched launch: CSR choices with smaller row blocks run with this block size (0 = as chosen)")
-    ap.add_argument("--queues", type=int, default=8, choices=(1, 8),
-                    help="batched launch: work queues (8 = one per XCD, 1 = shared)")
+    ap.add_argument("--queues", type=int, default=256,
+                    help="batched launch: ticket counters the units are dealt over (a power of two)")
     ap.add_argument("--e2e", action="store_true", help="end-to-end streaming sample of the held-out workload")
     ap.add_argument("--e2e-count", type=int, default=32, help="matrices of the streaming sample")
     ap.add_argument("--e2e-chunk", type=int, default=8, help="matrices per chunk of the streaming sample")

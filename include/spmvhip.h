@@ -177,22 +177,23 @@ void spmvhip_destroy(spmvhip_format *f);
 /* ---- batched SpMV (no counterpart in the reference, whose driver runs one
  * matrix per process, run.sh:116-151) ----
  * Many converted matrices, each with its own device x and y, computed by ONE
- * persistent kernel launch: the resident wavefronts sweep the list of all
- * members' units (CSR row blocks and long-row segments, SELL slices) in
- * order, wave w taking units w, w + W, ...; the last units are work items
- * drawn from atomic-ticket queues; long-row segments are completed by a
- * second kernel of the same sequence. With use_graph the sequence (ticket
- * reset + kernel + segment finalize) is captured once into a hipGraph and
- * replayed. Members must be CSR_VEC (block_nnz <= 512), SELL or SELL_C_S
- * handles, must outlive the batch and must not run elsewhere meanwhile.
- * tile_units: units per work item of the ticketed tail (0 = 4, at most 65536). */
+ * persistent kernel launch: the resident wavefronts draw the units of all
+ * members (CSR row blocks and long-row segments, SELL slices) in list order
+ * by atomic ticket, unit g from counter g % nqueues, each wave requesting its
+ * next ticket before it runs the current unit; long-row segments are
+ * completed by a second kernel of the same sequence. With use_graph the
+ * sequence (counter reset + kernel + segment finalize) is captured once into
+ * a hipGraph and replayed. Members must be CSR_VEC (block_nnz <= 512), SELL or
+ * SELL_C_S handles, must outlive the batch and must not run elsewhere
+ * meanwhile. tile_units: consecutive units per ticket (0 = 1, at most 65536). */
 typedef struct spmvhip_batch spmvhip_batch;
 int spmvhip_batch_create(int tile_units, spmvhip_batch **out);
-/* Tail queues (before finalize): 8 (default) = one per XCD, items dealt
- * round-robin; 1 = one shared queue. */
+/* Ticket counters (before finalize): a power of two, 1..1024 (default 256),
+ * each on its own 256-byte line; reduced at finalize to at most the number
+ * of resident wavefronts. */
 int spmvhip_batch_set_queues(spmvhip_batch *b, int nqueues);
 /* Persistent grid (before finalize): workgroups of 4 wavefronts per CU, 1..8,
- * at most what the kernel's resources allow; 0 (default) = 4. */
+ * at most what the kernel's resources allow; 0 (default) = the measured best. */
 int spmvhip_batch_set_grid(spmvhip_batch *b, int workgroups_per_cu);
 int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d_x, double *d_y);
 int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph);

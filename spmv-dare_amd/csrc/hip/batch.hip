@@ -11,19 +11,19 @@
 // order, so a heavy-tailed row is spread over many waves).
 //
 // Schedule. All units of all members form one list (members in insertion
-// order, each member's segments before its other units). The resident waves
-// (W of them) first sweep the list statically: wave w takes units w, w + W,
-// w + 2W, ..., so at any time the waves work on a window of about W adjacent
-// units -- the same in-order locality (x lines in L2, open DRAM pages) that
-// hardware dispatch gives a one-wave-per-unit launch, with no per-unit
-// bookkeeping. The last units (the tail, where per-wave differences in unit
-// cost would leave waves idle) form small work items on queues drawn from by
-// atomic tickets: one queue per XCD (workgroup b runs on XCD b % 8 and draws
-// from queue b % 8 only; the items are dealt round-robin, so the queues hold
-// the same work to within one item), or one shared queue. The tickets are
-// device-scope atomics, performed at the memory side: each counter has a
-// 256-byte line of its own, and a wave takes exactly one failing ticket, so
-// the launch's end does not queue tens of thousands of atomics on one line.
+// order, each member's segments before its other units), dealt over NC
+// ticket counters: unit g belongs to counter g % NC, and ticket t of counter
+// c is unit t * NC + c (or the `tile` units from there). Each resident wave
+// draws from one counter and requests its next ticket before it runs the
+// current unit, so the atomic's latency hides behind the unit. Because
+// tickets are handed out in order, the units in flight at any moment are a
+// window of about W adjacent units (W = resident waves), the in-order
+// locality of hardware dispatch. A static sweep (wave w: units w, w + W, ...)
+// lets the waves drift apart and widens that window, which raises the
+// memory latency every load sees (measured: 792 instead of 686 cycles per
+// L1-to-L2 read); tickets over one shared counter serialise on one line.
+// The counters are device-scope atomics, performed at the memory side, each
+// on a 256-byte line of its own.
 // Correctness does not depend on the dispatch order, only locality does.
 
 #include <algorithm>
@@ -35,14 +35,17 @@
 namespace spmvhip {
 
 constexpr int kBatchBnnzMax = 512; // largest csr_vec block_nnz a member may have
-constexpr int kTailItemUnits = 4;  // default units per work item of the dynamic tail
-constexpr int kTailRounds = 4;     // the tail is the last kTailRounds * W units
-constexpr int kCounterStride = 64; // ints between queue counters: one 256-byte line each
-// Default resident workgroups per CU. The persistent sweep is faster at 4
-// (16 waves per CU) than at the 8 that fit: 9.56 ms against 10.04 ms for
-// 2,048 generated matrices (2.57 G nonzeros), and 0.87 ms against 0.92 ms
-// for an 8M-row banded matrix; 3 per CU is slower again (10.46 ms).
-constexpr int kGridPerCu = 4;
+constexpr int kCounterStride = 64; // ints between ticket counters: one 256-byte line each
+// Ticket counters. Atomics on one counter serialise (about 12 ns each, as
+// measured: 500k units over 8 counters take 0.84 ms, over one 6.1 ms); from
+// 64 counters on, one 128M-nonzero matrix runs in the batch as fast as in
+// its own launch (0.36 ms).
+constexpr int kCountersDefault = 256;
+constexpr int kCountersMax = 1024;
+// Default resident workgroups per CU: all that fit. With per-unit tickets
+// the full grid is the fastest measured (9.26 ms at 8 and 7 per CU against
+// 9.38 at 6 and 10.5 at 4, for 2,048 generated matrices).
+constexpr int kGridPerCu = 8;
 
 // A member's device arrays, as pointer values (common.hpp: global_ptr). The
 // fields every unit reads come first, so that a wave fetches them with one
@@ -76,20 +79,24 @@ struct alignas(8) UnitRec {
 	int j0, j1;
 };
 
-struct WorkItem {
-	int g0, g1; // a range of the global unit list
-};
-
 struct SplitRef {
 	int mat, i; // split row i of member mat
 };
 
-// zero the kXcds queue counters (a kernel, not a memset: the batch is
+// zero the nc ticket counters (a kernel, not a memset: the batch is
 // captured into a hipGraph and replayed)
-__global__ void batch_reset(int *counter)
+__global__ void batch_reset(int *counter, int nc)
 {
-	if (threadIdx.x < kXcds)
+	if ((int) threadIdx.x < nc)
 		counter[threadIdx.x * kCounterStride] = 0;
+}
+
+__device__ __forceinline__ int batch_ticket(int *ctr)
+{
+	int t = 0;
+	if (lane_id() == 0)
+		t = atomicAdd(ctr, 1);
+	return t;
 }
 
 __global__ void batch_fill_csr(const int32_t *__restrict__ blocks, const int32_t *__restrict__ rp, int nb, int mk,
@@ -138,40 +145,35 @@ __device__ __forceinline__ void batch_unit(const BatchMat *__restrict__ mats, co
 	}
 }
 
-// units[0, nstatic): static sweep; items[qstart[q] .. qstart[q + 1]) is tail
-// queue q. BNNZ: LDS staging per wave for CSR row blocks, at least the
-// largest member block_nnz.
+// nc (a power of two, at most the resident wave count) ticket counters;
+// `tile` units per ticket. BNNZ: LDS staging per wave for CSR row blocks, at
+// least the largest member block_nnz.
 //
-// Unit indices are 32-bit (finalize checks the count): the loop bound is then
+// Unit indices are 32-bit (finalize checks the count): loop bounds are then
 // compared on the scalar unit. A 64-bit signed compare has no scalar form, and
 // the VGPRs it borrowed at the loop latch made the compiler wait there for
 // every outstanding memory operation of the wave, the y stores included.
 // The kernel holds all unit kinds' code; it is kept to 64 VGPRs so that 8
-// waves per SIMD stay resident.
+// waves per SIMD can stay resident.
 template <int BNNZ>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) batch_kernel(const BatchMat *__restrict__ mats,
-		const UnitRec *__restrict__ units, int nstatic, const WorkItem *__restrict__ items,
-		const int *__restrict__ qstart, int nqueues, int *__restrict__ counter)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) batch_kernel(
+		const BatchMat *__restrict__ mats, const UnitRec *__restrict__ units, int nunits, int tile, int nc,
+		int *__restrict__ counter)
 {
 	__shared__ double prod[kWavesPerBlock][2 * BNNZ];
 	const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-	const int W = (int) gridDim.x * kWavesPerBlock;
-	for (int g = (int) blockIdx.x * kWavesPerBlock + wv; g < nstatic; g += W)
-		batch_unit<BNNZ>(mats, units[g], prod[wv]);
-	const int q = (int) (blockIdx.x % nqueues);
-	const int q0 = qstart[q], q1 = qstart[q + 1];
-	if (q0 == q1)
-		return;
+	const int c = ((int) blockIdx.x * kWavesPerBlock + wv) & (nc - 1);
+	int *ctr = counter + c * kCounterStride;
+	int t = __builtin_amdgcn_readfirstlane(__shfl(batch_ticket(ctr), 0, kWave));
 	for (;;) {
-		int t = 0;
-		if (lane_id() == 0)
-			t = atomicAdd(counter + q * kCounterStride, 1);
-		t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, kWave));
-		if (q0 + t >= q1)
-			break; // queue drained: wave-uniform exit, every wave reaches it
-		const WorkItem wi = items[q0 + t];
-		for (int g = wi.g0; g < wi.g1; g++)
+		const long g0 = ((long) t * nc + c) * tile;
+		if (g0 >= nunits)
+			break; // wave-uniform exit: tickets only grow, every wave reaches it
+		const int tn = batch_ticket(ctr); // the next ticket, in flight during this one's units
+		const int g1 = (int) std::min<long>(g0 + tile, nunits);
+		for (int g = (int) g0; g < g1; g++)
 			batch_unit<BNNZ>(mats, units[g], prod[wv]);
+		t = __builtin_amdgcn_readfirstlane(__shfl(tn, 0, kWave));
 	}
 }
 
@@ -203,19 +205,17 @@ struct UnitSpan {
 } // namespace spmvhip
 
 struct spmvhip_batch {
-	int tile = spmvhip::kTailItemUnits; // units per work item of the dynamic tail
-	int bnnz = 256;                     // LDS staging of the kernel instance (256 or 512)
-	int queues = 8;                     // tail queues: 8 = one per XCD, 1 = shared
+	int tile = 1;                        // units per ticket
+	int bnnz = 256;                      // LDS staging of the kernel instance (256 or 512)
+	int queues = spmvhip::kCountersDefault; // ticket counters (a power of two)
+	int nc = 0;                          // counters in use (queues, at most the resident waves)
 	int wgs_per_cu = 0;                 // persistent grid: workgroups per CU (0 = kGridPerCu)
 	std::vector<spmvhip_format *> fmts;
 	std::vector<spmvhip::BatchMat> host_mats;
 	std::vector<spmvhip::UnitSpan> spans;
 	long nunits = 0;
-	std::vector<spmvhip::WorkItem> host_items;
 	std::vector<spmvhip::SplitRef> host_splits;
-	std::vector<int> host_qstart;
-	int nstatic = 0;
-	spmvhip::DevBuf mats, units, items, qstart, counter, splits;
+	spmvhip::DevBuf mats, units, counter, splits;
 	hipStream_t stream = nullptr;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
@@ -230,15 +230,15 @@ namespace {
 
 int batch_launch(spmvhip_batch *b, hipStream_t s)
 {
-	batch_reset<<<1, kWave, 0, s>>>(b->counter.as<int>());
+	batch_reset<<<1, kCountersMax, 0, s>>>(b->counter.as<int>(), b->nc);
 	SPMVHIP_CHECK(hipGetLastError());
 	if (b->nunits > 0) {
 		if (b->bnnz > 256)
 			batch_kernel<kBatchBnnzMax><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->units.as<UnitRec>(),
-					b->nstatic, b->items.as<WorkItem>(), b->qstart.as<int>(), b->queues, b->counter.as<int>());
+					(int) b->nunits, b->tile, b->nc, b->counter.as<int>());
 		else
 			batch_kernel<256><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->units.as<UnitRec>(),
-					b->nstatic, b->items.as<WorkItem>(), b->qstart.as<int>(), b->queues, b->counter.as<int>());
+					(int) b->nunits, b->tile, b->nc, b->counter.as<int>());
 		SPMVHIP_CHECK(hipGetLastError());
 	}
 	if (!b->host_splits.empty()) {
@@ -280,8 +280,8 @@ int spmvhip_batch_set_queues(spmvhip_batch *b, int nqueues)
 		return set_error(SPMVHIP_EINVAL, "NULL batch");
 	if (b->finalized)
 		return set_error(SPMVHIP_EINVAL, "batch already finalized");
-	if (nqueues != 1 && nqueues != kXcds)
-		return set_error(SPMVHIP_EINVAL, "batch: %d queues (1 shared, or %d per XCD)", nqueues, kXcds);
+	if (nqueues < 1 || nqueues > kCountersMax || (nqueues & (nqueues - 1)))
+		return set_error(SPMVHIP_EINVAL, "batch: %d ticket counters (a power of two, 1..%d)", nqueues, kCountersMax);
 	b->queues = nqueues;
 	return SPMVHIP_OK;
 }
@@ -371,8 +371,7 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 	if (b->finalized)
 		return set_error(SPMVHIP_EINVAL, "batch already finalized");
 	// persistent grid: wgs_per_cu resident workgroups per CU (at most what
-	// fits), a multiple of the XCD count so that every tail queue has the
-	// same number
+	// fits), a multiple of the XCD count
 	int dev = 0, cus = 0, per_cu = 0;
 	SPMVHIP_CHECK(hipGetDevice(&dev));
 	SPMVHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -384,30 +383,19 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 	long g = (long) cus * std::max(1, std::min(per_cu, want));
 	g = std::max<long>(kXcds, g / kXcds * kXcds);
 	b->grid = (int) g;
-	// the static sweep takes all but the last kTailRounds * W units; the tail
-	// becomes items of `tile` units dealt round-robin to the queues
+	// ticket counters: the requested number, at most the resident waves (a
+	// counter without a wave would leave its units undone)
 	const long N = b->nunits, W = g * kWavesPerBlock;
-	if (N + W > INT32_MAX)
+	if (N + (long) b->tile * kCountersMax > INT32_MAX)
 		return set_error(SPMVHIP_EINVAL, "batch: %ld work units (the kernel indexes them with 32 bits)", N);
-	b->nstatic = (int) std::max<long>(0, N - kTailRounds * W);
-	std::vector<std::vector<WorkItem>> q(kXcds);
-	long k = 0;
-	for (long g0 = b->nstatic; g0 < N; g0 += b->tile, k++)
-		q[b->queues == 1 ? 0 : k % kXcds].push_back({(int) g0, (int) std::min(N, g0 + b->tile)});
-	b->host_items.clear();
-	b->host_qstart.assign(kXcds + 1, 0);
-	for (int i = 0; i < kXcds; i++) {
-		b->host_qstart[i] = (int) b->host_items.size();
-		b->host_items.insert(b->host_items.end(), q[i].begin(), q[i].end());
-	}
-	b->host_qstart[kXcds] = (int) b->host_items.size();
+	b->nc = b->queues;
+	while (b->nc > W)
+		b->nc >>= 1;
 
 	const size_t nm = b->host_mats.size();
 	SPMVHIP_TRY(b->mats.alloc(std::max<size_t>(1, nm) * sizeof(BatchMat)));
 	SPMVHIP_TRY(b->units.alloc(std::max<long>(1, N) * sizeof(UnitRec)));
-	SPMVHIP_TRY(b->items.alloc(std::max<size_t>(1, b->host_items.size()) * sizeof(WorkItem)));
-	SPMVHIP_TRY(b->qstart.alloc((kXcds + 1) * sizeof(int)));
-	SPMVHIP_TRY(b->counter.alloc(kXcds * kCounterStride * sizeof(int)));
+	SPMVHIP_TRY(b->counter.alloc(kCountersMax * kCounterStride * sizeof(int)));
 	if (nm)
 		SPMVHIP_CHECK(hipMemcpy(b->mats.p, b->host_mats.data(), nm * sizeof(BatchMat), hipMemcpyHostToDevice));
 	// the unit records, from each member's device tables
@@ -426,10 +414,6 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 		SPMVHIP_CHECK(hipGetLastError());
 	}
 	SPMVHIP_CHECK(hipStreamSynchronize(b->stream));
-	if (!b->host_items.empty())
-		SPMVHIP_CHECK(hipMemcpy(b->items.p, b->host_items.data(), b->host_items.size() * sizeof(WorkItem),
-				hipMemcpyHostToDevice));
-	SPMVHIP_CHECK(hipMemcpy(b->qstart.p, b->host_qstart.data(), (kXcds + 1) * sizeof(int), hipMemcpyHostToDevice));
 	if (!b->host_splits.empty()) {
 		SPMVHIP_TRY(b->splits.alloc(b->host_splits.size() * sizeof(SplitRef)));
 		SPMVHIP_CHECK(hipMemcpy(b->splits.p, b->host_splits.data(), b->host_splits.size() * sizeof(SplitRef),

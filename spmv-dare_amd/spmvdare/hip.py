@@ -286,7 +286,7 @@ class Batch:
         self._keep = []
 
     def set_queues(self, nqueues):
-        """8 (default): one work queue per XCD; 1: one shared queue."""
+        """Ticket counters the units are dealt over (a power of two, default 256)."""
         _check(_lib.spmvhip_batch_set_queues(self._h, nqueues), "batch_set_queues")
 
     def set_grid(self, workgroups_per_cu):

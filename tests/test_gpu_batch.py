@@ -1,4 +1,4 @@
-"""Batched SpMV: one persistent, work-queue-scheduled launch over many
+"""Batched SpMV: one persistent, ticket-scheduled launch over many
 generated matrices of uneven sizes, directly and replayed from a hipGraph.
 Every member's y is checked against the oracle (normwise <= 1e-13) and, for
 SELL members, bit-identical to the member's own standalone launch."""
@@ -94,7 +94,7 @@ def test_batch_with_block512_member(members):
     b.destroy()
 
 
-@pytest.mark.parametrize("use_graph,queues", [(False, 8), (True, 8), (True, 1)])
+@pytest.mark.parametrize("use_graph,queues", [(False, 64), (True, 8), (True, 1), (True, 1024)])
 def test_batch_split_rows(members, use_graph, queues):
     """Members whose long rows are split into segments (split_nnz 256): the
     segments run as work items of the persistent kernel and the finalize
@@ -131,8 +131,8 @@ def test_batch_split_rows(members, use_graph, queues):
 
 @pytest.mark.parametrize("wgs", [1, 3, 8])
 def test_batch_grid_sizes(members, wgs):
-    """Any resident grid (workgroups per CU) computes the same y: the static
-    sweep and the ticketed tail cover every unit exactly once."""
+    """Any resident grid (workgroups per CU) computes the same y: the tickets
+    cover every unit exactly once."""
     b = hip.Batch(2)
     b.set_grid(wgs)
     for mb in members:
@@ -148,6 +148,9 @@ def test_batch_grid_sizes(members, wgs):
     bad = hip.Batch()
     with pytest.raises(hip.SpmvHipError):
         bad.set_grid(9)
+    for nq in (0, 3, 2048):
+        with pytest.raises(hip.SpmvHipError):
+            bad.set_queues(nq)
     bad.destroy()
 
 
