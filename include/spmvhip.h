@@ -195,6 +195,11 @@ int spmvhip_batch_set_queues(spmvhip_batch *b, int nqueues);
 /* Persistent grid (before finalize): workgroups of 4 wavefronts per CU, 1..8,
  * at most what the kernel's resources allow; 0 (default) = the measured best. */
 int spmvhip_batch_set_grid(spmvhip_batch *b, int workgroups_per_cu);
+/* XCD ranges (before finalize): 1 = the unit list is cut into 8 contiguous
+ * ranges of equal nonzero weight, XCD q's workgroups draw only from range q
+ * (queues / 8 counters each), so a member's x is fetched into one or two
+ * XCDs' L2; 0 (default) = every wave draws from the whole list. */
+int spmvhip_batch_set_xcd_ranges(spmvhip_batch *b, int on);
 int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d_x, double *d_y);
 int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph);
 int spmvhip_batch_run(spmvhip_batch *b);          /* asynchronous on the batch's stream */

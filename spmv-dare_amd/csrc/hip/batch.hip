@@ -27,6 +27,7 @@
 // Correctness does not depend on the dispatch order, only locality does.
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "format.hpp"
@@ -145,6 +146,11 @@ __device__ __forceinline__ void batch_unit(const BatchMat *__restrict__ mats, co
 	}
 }
 
+// The unit list's ranges: one (the whole list) or one per XCD.
+struct UnitRanges {
+	int b[kXcds + 1];
+};
+
 // nc (a power of two, at most the resident wave count) ticket counters;
 // `tile` units per ticket. BNNZ: LDS staging per wave for CSR row blocks, at
 // least the largest member block_nnz.
@@ -157,20 +163,24 @@ __device__ __forceinline__ void batch_unit(const BatchMat *__restrict__ mats, co
 // waves per SIMD can stay resident.
 template <int BNNZ>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) batch_kernel(
-		const BatchMat *__restrict__ mats, const UnitRec *__restrict__ units, int nunits, int tile, int nc,
-		int *__restrict__ counter)
+		const BatchMat *__restrict__ mats, const UnitRec *__restrict__ units, UnitRanges rg, int nranges, int tile,
+		int nc, int *__restrict__ counter)
 {
 	__shared__ double prod[kWavesPerBlock][2 * BNNZ];
 	const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-	const int c = ((int) blockIdx.x * kWavesPerBlock + wv) & (nc - 1);
-	int *ctr = counter + c * kCounterStride;
+	// workgroup b runs on XCD b % 8 (the grid is a multiple of 8)
+	const int q = nranges == 1 ? 0 : (int) (blockIdx.x % kXcds);
+	const int lw = (nranges == 1 ? (int) blockIdx.x : (int) (blockIdx.x / kXcds)) * kWavesPerBlock + wv;
+	const int c = lw & (nc - 1);
+	const int beg = rg.b[q], end = rg.b[q + 1];
+	int *ctr = counter + (q * nc + c) * kCounterStride;
 	int t = __builtin_amdgcn_readfirstlane(__shfl(batch_ticket(ctr), 0, kWave));
 	for (;;) {
-		const long g0 = ((long) t * nc + c) * tile;
-		if (g0 >= nunits)
+		const long g0 = beg + ((long) t * nc + c) * tile;
+		if (g0 >= end)
 			break; // wave-uniform exit: tickets only grow, every wave reaches it
 		const int tn = batch_ticket(ctr); // the next ticket, in flight during this one's units
-		const int g1 = (int) std::min<long>(g0 + tile, nunits);
+		const int g1 = (int) std::min<long>(g0 + tile, end);
 		for (int g = (int) g0; g < g1; g++)
 			batch_unit<BNNZ>(mats, units[g], prod[wv]);
 		t = __builtin_amdgcn_readfirstlane(__shfl(tn, 0, kWave));
@@ -200,6 +210,7 @@ struct UnitSpan {
 	int mk;
 	long off, count;
 	const int32_t *blocks, *rp, *seg_desc; // CSR blocks / segments (device)
+	double weight;                         // nonzeros per unit (the member's mean)
 };
 
 } // namespace spmvhip
@@ -208,7 +219,10 @@ struct spmvhip_batch {
 	int tile = 1;                        // units per ticket
 	int bnnz = 256;                      // LDS staging of the kernel instance (256 or 512)
 	int queues = spmvhip::kCountersDefault; // ticket counters (a power of two)
-	int nc = 0;                          // counters in use (queues, at most the resident waves)
+	int nc = 0;                          // counters per range in use (at most the range's resident waves)
+	bool xcd_ranges = false;             // one contiguous, nnz-balanced range of units per XCD
+	spmvhip::UnitRanges ranges{};
+	int nranges = 1;
 	int wgs_per_cu = 0;                 // persistent grid: workgroups per CU (0 = kGridPerCu)
 	std::vector<spmvhip_format *> fmts;
 	std::vector<spmvhip::BatchMat> host_mats;
@@ -230,15 +244,15 @@ namespace {
 
 int batch_launch(spmvhip_batch *b, hipStream_t s)
 {
-	batch_reset<<<1, kCountersMax, 0, s>>>(b->counter.as<int>(), b->nc);
+	batch_reset<<<1, kCountersMax, 0, s>>>(b->counter.as<int>(), b->nc * b->nranges);
 	SPMVHIP_CHECK(hipGetLastError());
 	if (b->nunits > 0) {
 		if (b->bnnz > 256)
 			batch_kernel<kBatchBnnzMax><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->units.as<UnitRec>(),
-					(int) b->nunits, b->tile, b->nc, b->counter.as<int>());
+					b->ranges, b->nranges, b->tile, b->nc, b->counter.as<int>());
 		else
 			batch_kernel<256><<<b->grid, kBlock, 0, s>>>(b->mats.as<BatchMat>(), b->units.as<UnitRec>(),
-					(int) b->nunits, b->tile, b->nc, b->counter.as<int>());
+					b->ranges, b->nranges, b->tile, b->nc, b->counter.as<int>());
 		SPMVHIP_CHECK(hipGetLastError());
 	}
 	if (!b->host_splits.empty()) {
@@ -283,6 +297,16 @@ int spmvhip_batch_set_queues(spmvhip_batch *b, int nqueues)
 	if (nqueues < 1 || nqueues > kCountersMax || (nqueues & (nqueues - 1)))
 		return set_error(SPMVHIP_EINVAL, "batch: %d ticket counters (a power of two, 1..%d)", nqueues, kCountersMax);
 	b->queues = nqueues;
+	return SPMVHIP_OK;
+}
+
+int spmvhip_batch_set_xcd_ranges(spmvhip_batch *b, int on)
+{
+	if (!b)
+		return set_error(SPMVHIP_EINVAL, "NULL batch");
+	if (b->finalized)
+		return set_error(SPMVHIP_EINVAL, "batch already finalized");
+	b->xcd_ranges = on != 0;
 	return SPMVHIP_OK;
 }
 
@@ -347,18 +371,19 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 	b->host_mats.push_back(M);
 	b->fmts.push_back(const_cast<spmvhip_format *>(f));
 	// the member's segments (its longest work) first, then its units
+	const double w = (double) f->nnz / (double) std::max(1L, units + nsegs);
 	if (nsegs > 0) {
 		b->spans.push_back({mi | kUnitSeg << kKindShift, b->nunits, nsegs, nullptr, nullptr,
-				f->seg_desc.as<int32_t>()});
+				f->seg_desc.as<int32_t>(), w});
 		b->nunits += nsegs;
 		for (long i = 0; i < f->nsplit; i++)
 			b->host_splits.push_back({mi, (int) i});
 	}
 	if (kind == kUnitCsr)
 		b->spans.push_back({mi | kind << kKindShift, b->nunits, units, f->blocks.as<int32_t>(),
-				f->row_ptr.as<int32_t>(), nullptr});
+				f->row_ptr.as<int32_t>(), nullptr, w});
 	else
-		b->spans.push_back({mi | kind << kKindShift, b->nunits, units, nullptr, nullptr, nullptr});
+		b->spans.push_back({mi | kind << kKindShift, b->nunits, units, nullptr, nullptr, nullptr, w});
 	b->nunits += units;
 	b->total_nnz += f->nnz;
 	return SPMVHIP_OK;
@@ -388,9 +413,33 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 	const long N = b->nunits, W = g * kWavesPerBlock;
 	if (N + (long) b->tile * kCountersMax > INT32_MAX)
 		return set_error(SPMVHIP_EINVAL, "batch: %ld work units (the kernel indexes them with 32 bits)", N);
-	b->nc = b->queues;
-	while (b->nc > W)
+	b->nranges = b->xcd_ranges ? kXcds : 1;
+	b->nc = std::max(1, b->queues / b->nranges);
+	while (b->nc > W / b->nranges)
 		b->nc >>= 1;
+	b->ranges = UnitRanges{};
+	for (int q = 0; q <= kXcds; q++)
+		b->ranges.b[q] = (int) N;
+	b->ranges.b[0] = 0;
+	if (b->nranges > 1) {
+		// cut points of equal nonzero weight (each member's units weigh its
+		// mean nonzeros per unit)
+		double total = 0, acc = 0;
+		for (const UnitSpan &sp : b->spans)
+			total += sp.weight * (double) sp.count;
+		int q = 1;
+		for (const UnitSpan &sp : b->spans) {
+			const double span_w = sp.weight * (double) sp.count;
+			while (q < kXcds && acc + span_w >= total * q / kXcds) {
+				const double need = total * q / kXcds - acc;
+				const long k = sp.weight > 0 ? std::min(sp.count, (long) std::ceil(need / sp.weight)) : 0;
+				b->ranges.b[q++] = (int) (sp.off + k);
+			}
+			acc += span_w;
+		}
+		for (int i = 1; i <= kXcds; i++)
+			b->ranges.b[i] = std::max(b->ranges.b[i], b->ranges.b[i - 1]);
+	}
 
 	const size_t nm = b->host_mats.size();
 	SPMVHIP_TRY(b->mats.alloc(std::max<size_t>(1, nm) * sizeof(BatchMat)));

@@ -72,6 +72,7 @@ _sig = {
     "spmvhip_batch_create": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "spmvhip_batch_set_queues": ([_vp, ctypes.c_int], ctypes.c_int),
     "spmvhip_batch_set_grid": ([_vp, ctypes.c_int], ctypes.c_int),
+    "spmvhip_batch_set_xcd_ranges": ([_vp, ctypes.c_int], ctypes.c_int),
     "spmvhip_batch_add": ([_vp, _vp, _vp, _vp], ctypes.c_int),
     "spmvhip_batch_finalize": ([_vp, ctypes.c_int], ctypes.c_int),
     "spmvhip_batch_run": ([_vp], ctypes.c_int),
@@ -292,6 +293,10 @@ class Batch:
     def set_grid(self, workgroups_per_cu):
         """Resident workgroups per CU of the persistent launch (0 = default)."""
         _check(_lib.spmvhip_batch_set_grid(self._h, workgroups_per_cu), "batch_set_grid")
+
+    def set_xcd_ranges(self, on):
+        """Cut the unit list into one contiguous, nnz-balanced range per XCD."""
+        _check(_lib.spmvhip_batch_set_xcd_ranges(self._h, int(bool(on))), "batch_set_xcd_ranges")
 
     def add(self, fmt, dx, dy):
         _check(_lib.spmvhip_batch_add(self._h, fmt._h, dx.ptr, dy.ptr), "batch_add")

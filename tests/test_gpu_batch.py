@@ -154,6 +154,25 @@ def test_batch_grid_sizes(members, wgs):
     bad.destroy()
 
 
+@pytest.mark.parametrize("queues,tile", [(256, 1), (8, 3), (1024, 1)])
+def test_batch_xcd_ranges(members, queues, tile):
+    """With one contiguous unit range per XCD every unit is still computed
+    exactly once (queues / 8 counters per range)."""
+    b = hip.Batch(tile)
+    b.set_queues(queues)
+    b.set_xcd_ranges(True)
+    for mb in members:
+        b.add(mb["fmt"], mb["dx"], mb["dy"])
+    b.finalize(use_graph=True)
+    for rep in range(2):
+        for mb in members:
+            mb["dy"].upload(np.full(mb["m"], np.nan))
+        b.run()
+        b.synchronize()
+        check(members)
+    b.destroy()
+
+
 def test_batch_rejects_unbatchable(members):
     a = gen.generate(gen.GenParams.parse(GEN["test_txt"]["line"]), stats=False)
     b = hip.Batch()
