@@ -213,7 +213,13 @@ class Selector:
 
     @classmethod
     def load(cls, path):
+        """A trained selector. Its output slots are variant indices, so the
+        variant list it was trained on must be the current VARIANTS."""
         d = json.load(open(path))
+        have = [variant_name(i) for i in range(len(VARIANTS))]
+        if d.get("variants") != have:
+            raise ValueError("%s was trained on another variant list; retrain it "
+                             "(python -m spmvdare.train_selector)" % path)
         s = cls(tuple(d["hidden"]))
         s.mu = np.array(d["mu"])
         s.sd = np.array(d["sd"])
