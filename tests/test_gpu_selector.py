@@ -59,3 +59,16 @@ def test_trained_selector_beats_static_choice():
     print(json.dumps(res, indent=1))
     assert res["mlp_top1"]["mean"] < res["static_best_variant"]["mean"]
     assert res["mlp_top3_refined"]["mean"] <= 0.05
+
+
+def test_committed_selector_regret():
+    """The committed model (the one bench.py loads) on the held-out corpus:
+    its top choice beats the static best variant and its top-3 refined by
+    timing stays within the 5 % regret target."""
+    tr = np.load(os.path.join(MODELS, "corpus_train.npz"))
+    te = np.load(os.path.join(MODELS, "corpus_test.npz"))
+    sel = tuning.Selector.load(os.path.join(MODELS, "selector.json"))
+    res = evaluate(sel, te["features"], te["times"], tr["times"])
+    print(json.dumps(res, indent=1))
+    assert res["mlp_top1"]["mean"] < res["static_best_variant"]["mean"]
+    assert res["mlp_top3_refined"]["mean"] <= 0.05
