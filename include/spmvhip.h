@@ -198,7 +198,8 @@ int spmvhip_batch_set_grid(spmvhip_batch *b, int workgroups_per_cu);
 /* XCD ranges (before finalize): 1 = the unit list is cut into 8 contiguous
  * ranges of equal nonzero weight, XCD q's workgroups draw only from range q
  * (queues / 8 counters each), so a member's x is fetched into one or two
- * XCDs' L2; 0 (default) = every wave draws from the whole list. */
+ * XCDs' L2; 0 = every wave draws from the whole list; -1 (default) =
+ * automatic: ranges when no member holds more than 1/64 of the units. */
 int spmvhip_batch_set_xcd_ranges(spmvhip_batch *b, int on);
 int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d_x, double *d_y);
 int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph);

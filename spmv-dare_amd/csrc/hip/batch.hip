@@ -220,7 +220,8 @@ struct spmvhip_batch {
 	int bnnz = 256;                      // LDS staging of the kernel instance (256 or 512)
 	int queues = spmvhip::kCountersDefault; // ticket counters (a power of two)
 	int nc = 0;                          // counters per range in use (at most the range's resident waves)
-	bool xcd_ranges = false;             // one contiguous, nnz-balanced range of units per XCD
+	int xcd_ranges = -1;                 // one contiguous, nnz-balanced range of units per XCD: 1 on, 0 off, -1 auto
+	long max_member_units = 0;
 	spmvhip::UnitRanges ranges{};
 	int nranges = 1;
 	int wgs_per_cu = 0;                 // persistent grid: workgroups per CU (0 = kGridPerCu)
@@ -306,7 +307,9 @@ int spmvhip_batch_set_xcd_ranges(spmvhip_batch *b, int on)
 		return set_error(SPMVHIP_EINVAL, "NULL batch");
 	if (b->finalized)
 		return set_error(SPMVHIP_EINVAL, "batch already finalized");
-	b->xcd_ranges = on != 0;
+	if (on < -1 || on > 1)
+		return set_error(SPMVHIP_EINVAL, "batch: xcd_ranges %d (1 on, 0 off, -1 automatic)", on);
+	b->xcd_ranges = on;
 	return SPMVHIP_OK;
 }
 
@@ -385,6 +388,7 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 	else
 		b->spans.push_back({mi | kind << kKindShift, b->nunits, units, nullptr, nullptr, nullptr, w});
 	b->nunits += units;
+	b->max_member_units = std::max(b->max_member_units, units + nsegs);
 	b->total_nnz += f->nnz;
 	return SPMVHIP_OK;
 }
@@ -413,7 +417,12 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 	const long N = b->nunits, W = g * kWavesPerBlock;
 	if (N + (long) b->tile * kCountersMax > INT32_MAX)
 		return set_error(SPMVHIP_EINVAL, "batch: %ld work units (the kernel indexes them with 32 bits)", N);
-	b->nranges = b->xcd_ranges ? kXcds : 1;
+	// automatic: per-XCD ranges when every member is small against the batch
+	// (x locality per XCD, about 1 % on 2,048 small matrices); a member holding
+	// more than 1/64 of the units keeps its units spread over all XCDs (on one
+	// large random-placement matrix the ranges cost up to 7 %)
+	const bool ranges_on = b->xcd_ranges == 1 || (b->xcd_ranges == -1 && b->max_member_units * 64 <= N);
+	b->nranges = ranges_on ? kXcds : 1;
 	b->nc = std::max(1, b->queues / b->nranges);
 	while (b->nc > W / b->nranges)
 		b->nc >>= 1;

@@ -295,8 +295,10 @@ class Batch:
         _check(_lib.spmvhip_batch_set_grid(self._h, workgroups_per_cu), "batch_set_grid")
 
     def set_xcd_ranges(self, on):
-        """Cut the unit list into one contiguous, nnz-balanced range per XCD."""
-        _check(_lib.spmvhip_batch_set_xcd_ranges(self._h, int(bool(on))), "batch_set_xcd_ranges")
+        """One contiguous, nnz-balanced range of units per XCD: True / False,
+        or -1 for the automatic choice (the default)."""
+        _check(_lib.spmvhip_batch_set_xcd_ranges(self._h, -1 if on == -1 else int(bool(on))),
+               "batch_set_xcd_ranges")
 
     def add(self, fmt, dx, dy):
         _check(_lib.spmvhip_batch_add(self._h, fmt._h, dx.ptr, dy.ptr), "batch_add")
