@@ -86,7 +86,9 @@ typedef struct spmvhip_params {
 	                         2 staged in LDS per workgroup of 256 rows (needs sigma | 256 or no
 	                         sort); 3 stored in sorted order to a device buffer, then one
 	                         coalesced permutation pass y[r] = ys[inv[r]] (SELL_C_S) */
-	int reserved[6];
+	int div_acc_copies;   /* DIV_RF: copies of the LDS row accumulators the lanes add into (a power
+	                         of two, at most 64; 0 = up to 4 within 4 KiB for the largest packet) */
+	int reserved[5];
 } spmvhip_params;
 
 typedef struct spmvhip_format spmvhip_format;   /* opaque */

@@ -321,20 +321,63 @@ __device__ __forceinline__ uint32_t div_coord(const uint8_t *pk, int at, int cb,
 	return (uint32_t) (c >> row_bits);
 }
 
-template <int K, bool C4>
+//
+// A4L64: the packet's coordinates are exactly 4 bytes (so, from the header's
+// 4-byte alignment, every coordinate is dword-aligned) and it has 64 lanes.
+// The full steps then read each coordinate with one aligned LDS dword read
+// and each length byte at a compile-time stride, with immediate offsets.
+template <int K, bool C4, bool A4L64 = false>
 __device__ __forceinline__ void div_rf1_group(const uint8_t *pk, int coords_at, int lens_at, int lens0, int crd0_at,
-		int stream_at, int steps, int cnt, int L, int cb, uint32_t col_min, int row_bits,
+		int stream_at, int steps, int cnt, int dv, int L, int cb, uint32_t col_min, int row_bits,
 		const double *__restrict__ x, double *yw, int &cursor)
 {
 	const uint32_t cmask = cb >= 4 ? 0xffffffffu : (1u << (8 * cb)) - 1;
+	const int cstride = A4L64 ? kWave * 4 : L * cb;
+	const int lstride = A4L64 ? kWave : L;
 	uint64_t val = 0;
-	for (int p0 = 0; p0 < steps; p0 += K) {
+	// full steps: every lane below L has at least dv values, so the first
+	// dv / K * K of them need no validity test; addresses advance by
+	// constant strides and the K rows stay in registers between the passes
+	const int nfull = dv / K * K;
+	if (nfull > 0 && lane_id() < L) {
+		int at = crd0_at, la = lens0;
+		for (int p0 = 0; p0 < nfull; p0 += K) {
+			uint32_t row[K];
+			double xv[K];
+#pragma unroll
+			for (int k = 0; k < K; k++) {
+				uint32_t col;
+				if (A4L64) {
+					const uint32_t c = *(const uint32_t *) (pk + at + k * cstride);
+					row[k] = c & ((1u << row_bits) - 1);
+					col = c >> row_bits;
+				} else {
+					col = div_coord<K, C4>(pk, at + k * cstride, cb, cmask, row_bits, row[k]);
+				}
+				xv[k] = x[col_min + col];
+			}
+			at += K * cstride;
+#pragma unroll
+			for (int k = 0; k < K; k++) {
+				const int lb = pk[la + k * lstride];
+				const int len = lb & 15, tz = (lb >> 4) << 2;
+				val += lds_le_b32(pk, stream_at + cursor, len) << tz;
+				cursor += len;
+				double d;
+				__builtin_memcpy(&d, &val, 8);
+				atomicAdd(&yw[row[k]], d * xv[k]);
+			}
+			la += K * lstride;
+		}
+	}
+	// the rest (at most K + 1 values per lane), each value tested
+	for (int p0 = nfull; p0 < steps; p0 += K) {
 		double xv[K];
 #pragma unroll
 		for (int k = 0; k < K; k++) {
 			const bool ok = p0 + k < cnt;
 			uint32_t row;
-			const uint32_t col = div_coord<K, C4>(pk, ok ? crd0_at + (p0 + k) * L * cb : coords_at, cb, cmask,
+			const uint32_t col = div_coord<K, C4>(pk, ok ? crd0_at + (p0 + k) * cstride : coords_at, cb, cmask,
 					row_bits, row);
 			xv[k] = x[col_min + (ok ? col : 0u)];
 		}
@@ -342,7 +385,7 @@ __device__ __forceinline__ void div_rf1_group(const uint8_t *pk, int coords_at, 
 		for (int k = 0; k < K; k++) {
 			const bool ok = p0 + k < cnt;
 			uint32_t row;
-			(void) div_coord<K, C4>(pk, ok ? crd0_at + (p0 + k) * L * cb : coords_at, cb, cmask, row_bits, row);
+			(void) div_coord<K, C4>(pk, ok ? crd0_at + (p0 + k) * cstride : coords_at, cb, cmask, row_bits, row);
 			const int lb = pk[ok ? lens0 + (p0 + k) * L : lens_at];
 			const int len = ok ? lb & 15 : 0, tz = (lb >> 4) << 2;
 			val += lds_le_b32(pk, stream_at + cursor, len) << tz;
@@ -355,15 +398,37 @@ __device__ __forceinline__ void div_rf1_group(const uint8_t *pk, int coords_at, 
 	}
 }
 
+constexpr int kDivAccCopiesMax = 64;    // copies of the LDS row accumulators (a power of two)
+// The automatic choice: up to 4 copies within 4 KiB. Measured on the 8M-row
+// matrix (packets of at most 48 rows): 1 copy 0.89 ms, 4 copies 0.747 ms,
+// 8 copies 0.767 ms, 16 copies 0.89 ms, 32 copies 1.29 ms (more LDS per
+// workgroup, fewer resident waves, more zeroing and summing).
+constexpr int kDivAccCopiesAuto = 4;
+constexpr long kDivAccBudget = 4096;
+
+// row r summed over the nc accumulator copies, in a fixed order
+__device__ __forceinline__ double div_acc_sum(const double *yw, int stride, int nc, int r)
+{
+	double s = yw[r];
+	for (int c = 1; c < nc; c++)
+		s += yw[c * stride + r];
+	return s;
+}
+
 template <int K>
 __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__restrict__ bytes,
 		const int64_t *__restrict__ pk_off, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
-		long ngroups, int L, int buf_bytes, const double *__restrict__ x, double *__restrict__ y)
+		long ngroups, int L, int buf_bytes, int acc_stride, int acc_copies, const double *__restrict__ x,
+		double *__restrict__ y)
 {
 	extern __shared__ u32x4 smem[];
 	uint8_t *pk = (uint8_t *) smem;                   // buf_bytes (a multiple of 16, padding included)
-	double *yw = (double *) (pk + buf_bytes);          // row accumulators (the most rows of any packet)
+	double *yw = (double *) (pk + buf_bytes);          // acc_copies copies of the row accumulators
 	const int lane = lane_id();
+	// lane l adds into copy l % acc_copies: lanes whose values fall in the
+	// same row collide on one LDS address (and serialise) acc_copies times
+	// less often -- with one copy those collisions bound the decoder
+	double *ywl = yw + (lane & (acc_copies - 1)) * acc_stride;
 	const long g = blockIdx.x;
 	if (g >= ngroups)
 		return;
@@ -410,7 +475,7 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 		const long coords_at = 28 + 4 * (long) L + 8 * (long) nrfs;
 		const long lens_at = coords_at + (long) nv * cb;
 		const long stream_at = lens_at + nu + (incl - my_size);
-		for (uint32_t r = lane; r < nrows; r += kWave)
+		for (int r = lane; r < acc_copies * acc_stride; r += kWave)
 			yw[r] = 0.0;
 		wave_fence();
 		long cursor = 0, ks = 0, cs = 0;
@@ -423,12 +488,16 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 			if (rf == 1) {
 				int cur = (int) cursor;
 				const int lens0 = (int) (lens_at + ks) + lane, crd0 = (int) cs + lane;
-				if (cb <= 4)
+				if (cb == 4 && L == kWave)
+					div_rf1_group<K, true, true>(pk, (int) coords_at, (int) lens_at, lens0,
+							(int) coords_at + crd0 * cb, (int) stream_at, (int) steps, (int) cnt, (int) dv, L, cb,
+							col_min, row_bits, x, ywl, cur);
+				else if (cb <= 4)
 					div_rf1_group<K, true>(pk, (int) coords_at, (int) lens_at, lens0, (int) coords_at + crd0 * cb,
-							(int) stream_at, (int) steps, (int) cnt, L, cb, col_min, row_bits, x, yw, cur);
+							(int) stream_at, (int) steps, (int) cnt, (int) dv, L, cb, col_min, row_bits, x, ywl, cur);
 				else
 					div_rf1_group<K, false>(pk, (int) coords_at, (int) lens_at, lens0, (int) coords_at + crd0 * cb,
-							(int) stream_at, (int) steps, (int) cnt, L, cb, col_min, row_bits, x, yw, cur);
+							(int) stream_at, (int) steps, (int) cnt, (int) dv, L, cb, col_min, row_bits, x, ywl, cur);
 				cursor = cur;
 				ks += n;
 				cs += n;
@@ -452,7 +521,7 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 						for (long t = 0; t < rf; t++) {
 							const long cpos = pp < dv ? (pp * rf + t) * L + lane : mul_c + lane * rf + t;
 							const uint64_t cc = lds_le(pk, coords_at + (cs + cpos) * cb, cb);
-							atomicAdd(&yw[(uint32_t) (cc & row_mask)],
+							atomicAdd(&ywl[(uint32_t) (cc & row_mask)],
 									d * x[col_min + (uint32_t) (cc >> row_bits)]);
 						}
 					}
@@ -466,11 +535,11 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 			// the group's rows outside the packet are empty rows
 			for (long r = gr0 + lane; r < gr1; r += kWave) {
 				const long i = r - (long) row_min;
-				y[r] = i >= 0 && i < (long) nrows ? yw[i] : 0.0;
+				y[r] = i >= 0 && i < (long) nrows ? div_acc_sum(yw, acc_stride, acc_copies, (int) i) : 0.0;
 			}
 		} else {
 			for (uint32_t r = lane; r < nrows; r += kWave)
-				y[row_min + r] += yw[r];
+				y[row_min + r] += div_acc_sum(yw, acc_stride, acc_copies, (int) r);
 		}
 		wave_fence(); // the packet buffer and accumulators are reused
 	}
@@ -492,6 +561,10 @@ int div_convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col, co
 	const long P = f->p.div_packet_vals > 0 ? f->p.div_packet_vals : 16384;
 	if (L > kWave)
 		return set_error(SPMVHIP_EINVAL, "div_rf: %d lanes > the 64-lane wavefront", L);
+	const int ac = f->p.div_acc_copies;
+	if (ac < 0 || ac > kDivAccCopiesMax || (ac & (ac - 1)))
+		return set_error(SPMVHIP_EINVAL, "div_rf: div_acc_copies %d (0, or a power of two up to %d)", ac,
+				kDivAccCopiesMax);
 	if (T > 4096)
 		return set_error(SPMVHIP_EINVAL, "div_rf: %d partitions", T);
 	f->p.div_threads = T;
@@ -529,7 +602,16 @@ int div_convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col, co
 		memcpy(&nr, &st.bytes[st.pk_off[p] + 8], sizeof(nr));
 		max_rows = std::max(max_rows, nr);
 	}
-	f->div_acc_bytes = (int) ((max_rows * sizeof(double) + 15) / 16 * 16);
+	// an odd stride puts the copies of one row in different LDS banks
+	f->div_acc_stride = (int) (max_rows | 1);
+	int nc = f->p.div_acc_copies;
+	if (nc == 0) {
+		nc = 1;
+		while (nc < kDivAccCopiesAuto && 2 * nc * f->div_acc_stride * (long) sizeof(double) <= kDivAccBudget)
+			nc *= 2;
+	}
+	f->p.div_acc_copies = nc;
+	f->div_acc_bytes = (int) ((nc * f->div_acc_stride * sizeof(double) + 15) / 16 * 16);
 	f->div_buf_bytes = max_pk + 16 + f->div_acc_bytes <= kDivLdsMax ? (int) (max_pk + 16) : 0;
 	SPMVHIP_TRY(f->div_stream.alloc(dev.size()));
 	SPMVHIP_TRY(f->div_pk_off.alloc((np + 1) * sizeof(int64_t)));
@@ -565,15 +647,15 @@ int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 		if (f->p.unroll >= 16)
 			div_spmv_lds_kernel<16><<<(unsigned) f->div_ngroups, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
 					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
-					f->div_ngroups, f->div_lanes, f->div_buf_bytes, x, y);
+					f->div_ngroups, f->div_lanes, f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, x, y);
 		else if (f->p.unroll >= 8)
 			div_spmv_lds_kernel<8><<<(unsigned) f->div_ngroups, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
 					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
-					f->div_ngroups, f->div_lanes, f->div_buf_bytes, x, y);
+					f->div_ngroups, f->div_lanes, f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, x, y);
 		else
 			div_spmv_lds_kernel<4><<<(unsigned) f->div_ngroups, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
 					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
-					f->div_ngroups, f->div_lanes, f->div_buf_bytes, x, y);
+					f->div_ngroups, f->div_lanes, f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, x, y);
 	} else {
 		div_spmv_kernel<8><<<grid_for(f->div_ngroups, kDivWavesPerBlock), kBlock, 0, s>>>(f->div_stream.as<uint8_t>(),
 				f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),

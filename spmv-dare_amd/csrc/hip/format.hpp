@@ -47,7 +47,8 @@ struct spmvhip_format {
 	long div_npackets = 0, div_ngroups = 0, div_bytes = 0;
 	int div_lanes = 64;
 	int div_buf_bytes = 0;        // LDS packet buffer of the staged decoder: largest packet + 16, or 0 if too large
-	int div_acc_bytes = 2048;     // LDS row accumulators of the staged decoder: the most rows of any packet
+	int div_acc_bytes = 2048;     // LDS row accumulators of the staged decoder (all copies)
+	int div_acc_stride = 255;     // rows per accumulator copy: the most rows of any packet, odd
 
 	// host-pointer contract
 	const double *bound_x = nullptr;

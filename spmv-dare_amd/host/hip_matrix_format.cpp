@@ -114,6 +114,7 @@ struct Matrix_Format *csr_to_format(INT_T *row_ptr, INT_T *col_ind, ValueTypeRef
 	p.y_mode = (int) env_long("SELL_Y_MODE", 0);
 	p.div_threads = (int) env_long("DIV_THREADS", 0);
 	p.div_lanes = (int) env_long("DIV_LANES", 0);
+	p.div_acc_copies = (int) env_long("DIV_ACC_COPIES", 0);
 	p.div_packet_vals = env_long("CSRCV_NUM_PACKET_VALS", 0);
 
 	HipMatrixFormat *f = new HipMatrixFormat(m, n, nnz);

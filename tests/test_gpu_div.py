@@ -21,16 +21,51 @@ def device():
     hip.init(0)
 
 
+# matrices with wide column spans: their packets have 4-byte coordinates
+# (row bits rounded to 8 plus up to 24 column bits; the decoder's aligned
+# 64-lane path), 5-byte coordinates, or a mix of widths
+WIDE = {"wide_cols_4m": "20000 4000000 16 4 normal random 0.5 1.5 1 0.2 3",
+        "wide_cols_30m": "30000 30000000 24 12 gamma random 0.9 30 2 0.3 5",
+        "wide_diag_2m": "50000 2000000 20 8 gamma diagonal 0.2 10 3 0.4 9"}
+
+
 @pytest.mark.parametrize("name", NAMES)
 @pytest.mark.parametrize("T,L,P", CODECS)
 @pytest.mark.parametrize("unroll", [4, 8, 16])
 def test_div_device_decode_and_spmv(name, T, L, P, unroll):
     """unroll picks the decoder instance: 4, 8 or 16 values per lane and step."""
-    rp, ci, va, n = matrix(name)
+    check_div(*matrix(name), T, L, P, unroll, name)
+
+
+@pytest.mark.parametrize("name", sorted(WIDE))
+@pytest.mark.parametrize("T,L,P", [(1, 64, 16384), (3, 64, 300), (2, 64, 1024), (2, 32, 1024)])
+@pytest.mark.parametrize("unroll", [4, 8, 16])
+def test_div_wide_columns(name, T, L, P, unroll):
+    a = gen.generate(gen.GenParams.parse(WIDE[name]), stats=False)
+    check_div(a.row_ptr, a.col_ind, a.values, a.n, T, L, P, unroll, name)
+
+
+@pytest.mark.parametrize("name", ["wide_cols_4m", "wide_diag_2m"])
+@pytest.mark.parametrize("copies", [1, 2, 64])
+def test_div_accumulator_copies(name, copies):
+    """The row accumulators' copy count (automatic elsewhere) changes only
+    which LDS address a lane adds into."""
+    a = gen.generate(gen.GenParams.parse(WIDE[name]), stats=False)
+    check_div(a.row_ptr, a.col_ind, a.values, a.n, 2, 64, 1024, 8, name, div_acc_copies=copies)
+
+
+def test_div_accumulator_copies_rejected():
+    rp, ci, va, n = matrix("test_txt")
+    for bad in (3, 128, -1):
+        with pytest.raises(hip.SpmvHipError):
+            hip.csr_to_format(rp, ci, va, rp.size - 1, n, ci.size, kind="div_rf", div_acc_copies=bad)
+
+
+def check_div(rp, ci, va, n, T, L, P, unroll, name, **kw):
     m = rp.size - 1
-    print("case", name, T, L, P, unroll, flush=True)
+    print("case", name, T, L, P, unroll, kw, flush=True)
     f = hip.csr_to_format(rp, ci, va, m, n, ci.size, kind="div_rf", div_threads=T, div_lanes=L,
-                          div_packet_vals=P, unroll=unroll)
+                          div_packet_vals=P, unroll=unroll, **kw)
     if ci.size:
         b, off, grp = hip.div_encode(rp, ci, va, n, T=T, L=L, P=P)
         r, c, v = pyoracle.div_decode(b, L, ci.size)
