@@ -445,13 +445,15 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 		// copy the packet to LDS with direct global-to-LDS 16-byte loads (lane i
 		// of a round writes bytes [16 i, 16 i + 16) of the round's 1 KiB; lanes
 		// past the packet are masked off, so no write leaves the packet
-		// buffer), all rounds in flight at once, no VGPR staging
+		// buffer), all rounds in flight at once, no VGPR staging. The loads are
+		// non-temporal (cache policy 2 = nt): packets are read once, and the L2
+		// is left to the x lines the gathers reuse
 		const u32x4 *src = (const u32x4 *) (bytes + pk_off[p]);
 		const int n16 = (int) ((pk_off[p + 1] - pk_off[p] + 15) >> 4);
 		for (int j0 = 0; j0 < n16; j0 += kWave) {
 			if (j0 + lane < n16)
 				__builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *) (src + j0 + lane),
-						(__attribute__((address_space(3))) void *) (smem + j0), 16, 0, 0);
+						(__attribute__((address_space(3))) void *) (smem + j0), 16, 0, 2);
 		}
 		__builtin_amdgcn_s_waitcnt(0); // the copies have landed in LDS
 		if (lane == 0) // zero padding after the packet: the decode reads whole words
