@@ -54,6 +54,22 @@ def test_div_accumulator_copies(name, copies):
     check_div(a.row_ptr, a.col_ind, a.values, a.n, 2, 64, 1024, 8, name, div_acc_copies=copies)
 
 
+@pytest.mark.parametrize("name", ["cfg1", "batch_03", "dense_rows", "wide_cols_4m", "wide_diag_2m"])
+@pytest.mark.parametrize("T,L,P", [(2, 64, 1024), (3, 32, 300)])
+@pytest.mark.parametrize("unroll", [4, 16])
+def test_div_repeated_values(name, T, L, P, unroll):
+    """Values rounded to multiples of 1/2 repeat within packets, so every
+    packet holds replication-factor groups above 1 (the decoders' general
+    path, which the generated matrices' distinct values hardly reach), with
+    1- to 4-byte coordinates."""
+    if name in WIDE:
+        a = gen.generate(gen.GenParams.parse(WIDE[name]), stats=False)
+        rp, ci, va, n = a.row_ptr, a.col_ind, a.values, a.n
+    else:
+        rp, ci, va, n = matrix(name)
+    check_div(rp, ci, np.round(va * 2) / 2, n, T, L, P, unroll, name + "_halves")
+
+
 def test_div_accumulator_copies_rejected():
     rp, ci, va, n = matrix("test_txt")
     for bad in (3, 128, -1):
