@@ -36,7 +36,7 @@ CANDIDATES = [
     ("sell_c_s", {"sigma": 4096, "lanes_per_row": 4, "unroll": 4}),
     ("sell_c_s", {"sigma": 1}),
     ("sell", {"lanes_per_row": 4, "unroll": 4}),
-    ("div_rf", {"div_packet_vals": 1024, "div_threads": 64, "unroll": 8}),
+    ("div_rf", {"div_packet_vals": 1024, "div_threads": 64, "unroll": 8, "xcd_remap": 1}),
     ("csr_naive", {}),
 ]
 
@@ -131,7 +131,7 @@ def cpu_baseline_port(a, x, y_gpu, budget_s, cores):
 PROFILED = {  # the variant each profile summary's byte counters were taken on
     ("spmv8m", "csr_vec"): {"block_nnz": 256},
     ("spmv8m", "sell_c_s"): {"sigma": 256, "lanes_per_row": 4, "unroll": 4},
-    ("spmv8m", "div_rf"): {"div_packet_vals": 1024, "div_threads": 64, "unroll": 8},
+    ("spmv8m", "div_rf"): {"div_packet_vals": 1024, "div_threads": 64, "unroll": 8, "xcd_remap": 1},
 }
 
 

@@ -71,7 +71,10 @@ typedef struct spmvhip_params {
 	                         SELL*: lanes per row T (power of two, 0/1 = the reference layout) */
 	int block_nnz;        /* CSR_VEC: nnz per row block staged in LDS (64..1024, default 256) */
 	int split_nnz;        /* CSR_VEC: rows longer than this are split across waves (default 8192) */
-	int xcd_remap;        /* 1 = give each XCD a contiguous range of work (default 0) */
+	int xcd_remap;        /* CSR_VEC row blocks, DIV_RF packet groups (staged decoder): deal the work
+	                         to the 8 XCDs in chunks of consecutive workgroups, so each XCD's L2
+	                         serves one range of rows at a time. 1 = chunks of 64, n > 1 = chunks
+	                         of n, 0 = in order (default) */
 	int x_window;         /* SELL*: stage each workgroup's x window [min col, max col] of its
 	                         256 rows in LDS when it spans <= this many doubles (1 = 2048;
 	                         at most 16384); 0 = off */

@@ -30,6 +30,7 @@
 // value) sequence for the parity checks.
 
 #include "format.hpp"
+#include "kernels.hpp"
 #include "../div/div_codec.hpp"
 
 #include <algorithm>
@@ -418,7 +419,7 @@ __device__ __forceinline__ double div_acc_sum(const double *yw, int stride, int 
 template <int K>
 __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__restrict__ bytes,
 		const int64_t *__restrict__ pk_off, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
-		long ngroups, int L, int buf_bytes, int acc_stride, int acc_copies, const double *__restrict__ x,
+		long ngroups, int L, int buf_bytes, int acc_stride, int acc_copies, int xcd_k, const double *__restrict__ x,
 		double *__restrict__ y)
 {
 	extern __shared__ u32x4 smem[];
@@ -429,7 +430,7 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 	// same row collide on one LDS address (and serialise) acc_copies times
 	// less often -- with one copy those collisions bound the decoder
 	double *ywl = yw + (lane & (acc_copies - 1)) * acc_stride;
-	const long g = blockIdx.x;
+	const long g = xcd_logical_block(xcd_k); // groups dealt to XCDs in chunks of xcd_k (0: in order)
 	if (g >= ngroups)
 		return;
 	const int p_end = grp_pk[g + 1];
@@ -646,18 +647,20 @@ int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 	}
 	if (f->div_buf_bytes > 0) {
 		const size_t lds = (size_t) f->div_buf_bytes + (size_t) f->div_acc_bytes;
+		const int xk = xcd_chunk(f->p.xcd_remap);
+		const unsigned grid = xcd_padded_grid((unsigned) f->div_ngroups, xk);
 		if (f->p.unroll >= 16)
-			div_spmv_lds_kernel<16><<<(unsigned) f->div_ngroups, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
+			div_spmv_lds_kernel<16><<<grid, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
 					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
-					f->div_ngroups, f->div_lanes, f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, x, y);
+					f->div_ngroups, f->div_lanes, f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, xk, x, y);
 		else if (f->p.unroll >= 8)
-			div_spmv_lds_kernel<8><<<(unsigned) f->div_ngroups, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
+			div_spmv_lds_kernel<8><<<grid, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
 					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
-					f->div_ngroups, f->div_lanes, f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, x, y);
+					f->div_ngroups, f->div_lanes, f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, xk, x, y);
 		else
-			div_spmv_lds_kernel<4><<<(unsigned) f->div_ngroups, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
+			div_spmv_lds_kernel<4><<<grid, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
 					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
-					f->div_ngroups, f->div_lanes, f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, x, y);
+					f->div_ngroups, f->div_lanes, f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, xk, x, y);
 	} else {
 		div_spmv_kernel<8><<<grid_for(f->div_ngroups, kDivWavesPerBlock), kBlock, 0, s>>>(f->div_stream.as<uint8_t>(),
 				f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),

@@ -70,6 +70,19 @@ def test_div_repeated_values(name, T, L, P, unroll):
     check_div(rp, ci, np.round(va * 2) / 2, n, T, L, P, unroll, name + "_halves")
 
 
+@pytest.mark.parametrize("name", ["wide_diag_2m", "ragged_4097", "dense_rows"])
+@pytest.mark.parametrize("remap", [1, 3, 100000])
+def test_div_xcd_chunks(name, remap):
+    """Packet groups dealt to the XCDs in chunks (64, 3, or one chunk larger
+    than the whole grid) on a padded grid: every group is decoded once."""
+    if name in WIDE:
+        a = gen.generate(gen.GenParams.parse(WIDE[name]), stats=False)
+        rp, ci, va, n = a.row_ptr, a.col_ind, a.values, a.n
+    else:
+        rp, ci, va, n = matrix(name)
+    check_div(rp, ci, va, n, 2, 64, 1024, 8, name, xcd_remap=remap)
+
+
 def test_div_accumulator_copies_rejected():
     rp, ci, va, n = matrix("test_txt")
     for bad in (3, 128, -1):
