@@ -328,9 +328,9 @@ __device__ __forceinline__ uint32_t div_coord(const uint8_t *pk, int at, int cb,
 // The full steps then read each coordinate with one aligned LDS dword read
 // and each length byte at a compile-time stride, with immediate offsets.
 template <int K, bool C4, bool A4L64 = false>
-__device__ __forceinline__ void div_rf1_group(const uint8_t *pk, int coords_at, int lens_at, int lens0, int crd0_at,
-		int stream_at, int steps, int cnt, int dv, int L, int cb, uint32_t col_min, int row_bits,
-		const double *__restrict__ x, double *yw, int &cursor)
+__device__ __forceinline__ void div_rf1_group(const uint8_t *__restrict__ pk, int coords_at, int lens_at, int lens0,
+		int crd0_at, int stream_at, int steps, int cnt, int dv, int L, int cb, uint32_t col_min, int row_bits,
+		const double *__restrict__ x, double *__restrict__ yw, int &cursor)
 {
 	const uint32_t cmask = cb >= 4 ? 0xffffffffu : (1u << (8 * cb)) - 1;
 	const int cstride = A4L64 ? kWave * 4 : L * cb;
@@ -358,12 +358,27 @@ __device__ __forceinline__ void div_rf1_group(const uint8_t *pk, int coords_at, 
 				xv[k] = x[col_min + col];
 			}
 			at += K * cstride;
+			// the step's K length bytes, then its K deltas, are read before its
+			// first accumulation (the scheduling barriers keep them together):
+			// the compiler does not move loads across the LDS atomics, so reads
+			// between the atomics put one LDS round trip per value on the
+			// critical path.
+			int lb[K];
+#pragma unroll
+			for (int k = 0; k < K; k++)
+				lb[k] = pk[la + k * lstride];
+			__builtin_amdgcn_sched_barrier(0); // all length bytes requested together
+			uint64_t dlt[K];
 #pragma unroll
 			for (int k = 0; k < K; k++) {
-				const int lb = pk[la + k * lstride];
-				const int len = lb & 15, tz = (lb >> 4) << 2;
-				val += lds_le_b32(pk, stream_at + cursor, len) << tz;
+				const int len = lb[k] & 15, tz = (lb[k] >> 4) << 2;
+				dlt[k] = lds_le_b32(pk, stream_at + cursor, len) << tz;
 				cursor += len;
+			}
+			__builtin_amdgcn_sched_barrier(0); // then all deltas, before the first accumulation
+#pragma unroll
+			for (int k = 0; k < K; k++) {
+				val += dlt[k];
 				double d;
 				__builtin_memcpy(&d, &val, 8);
 				atomicAdd(&yw[row[k]], d * xv[k]);
