@@ -31,6 +31,7 @@ WORKLOADS = {
 }
 CANDIDATES = [
     ("csr_vec", {"block_nnz": 256}),
+    ("csr_vec", {"block_nnz": 256, "xcd_remap": 1}),
     ("sell_c_s", {"sigma": 256, "lanes_per_row": 4, "unroll": 4}),
     ("sell_c_s", {"sigma": 256}),
     ("sell_c_s", {"sigma": 4096, "lanes_per_row": 4, "unroll": 4}),

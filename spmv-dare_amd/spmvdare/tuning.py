@@ -1,8 +1,9 @@
 """Autotuner and learned format selector.
 
 Variant space: 40 format x tuning choices over the batchable drop-in formats
-(row-length-binned CSR with rows-per-wave / lanes / XCD mapping, SELL with
-its inner unroll, SELL-C-sigma with sigma and unroll). The exhaustive
+(row-length-binned CSR with rows-per-wave and lanes per row, SELL with
+lanes per row and its inner unroll, SELL-C-sigma with sigma, lanes per row,
+unroll and x windows). The exhaustive
 autotuner converts a matrix to every variant on the device and times it; the
 selector is an FP32 MLP 14 -> 64 -> 64 -> 40 (hand-written HIP kernels,
 spmvhip_mlp_*) that predicts every variant's slowdown log(t / t_best) from
