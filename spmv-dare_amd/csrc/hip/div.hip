@@ -322,6 +322,59 @@ __device__ __forceinline__ uint32_t div_coord(const uint8_t *pk, int at, int cb,
 	return (uint32_t) (c >> row_bits);
 }
 
+// S full steps of K values of one lane (every lane below L has them all):
+// first the S * K coordinates and their x gathers, then per step its K
+// length bytes, its K deltas and its K accumulations. The step's length
+// bytes, then its deltas, are read before its first accumulation (the
+// scheduling barriers keep them together): the compiler does not move loads
+// across the LDS atomics, so reads between the atomics would put one LDS
+// round trip per value on the critical path.
+template <int K, int S, bool C4, bool A4L64>
+__device__ __forceinline__ void div_full_steps(const uint8_t *__restrict__ pk, int &at, int &la, int cstride,
+		int lstride, int cb, uint32_t cmask, int row_bits, uint32_t col_min, int stream_at,
+		const double *__restrict__ x, double *__restrict__ yw, int &cursor, uint64_t &val)
+{
+	uint32_t row[S * K];
+	double xv[S * K];
+#pragma unroll
+	for (int k = 0; k < S * K; k++) {
+		uint32_t col;
+		if (A4L64) {
+			const uint32_t c = *(const uint32_t *) (pk + at + k * cstride);
+			row[k] = c & ((1u << row_bits) - 1);
+			col = c >> row_bits;
+		} else {
+			col = div_coord<K, C4>(pk, at + k * cstride, cb, cmask, row_bits, row[k]);
+		}
+		xv[k] = x[col_min + col];
+	}
+	at += S * K * cstride;
+#pragma unroll
+	for (int s = 0; s < S; s++) {
+		int lb[K];
+#pragma unroll
+		for (int k = 0; k < K; k++)
+			lb[k] = pk[la + k * lstride];
+		__builtin_amdgcn_sched_barrier(0); // all length bytes requested together
+		uint64_t dlt[K];
+#pragma unroll
+		for (int k = 0; k < K; k++) {
+			const int len = lb[k] & 15, tz = (lb[k] >> 4) << 2;
+			dlt[k] = lds_le_b32(pk, stream_at + cursor, len) << tz;
+			cursor += len;
+		}
+		__builtin_amdgcn_sched_barrier(0); // then all deltas, before the first accumulation
+#pragma unroll
+		for (int k = 0; k < K; k++) {
+			val += dlt[k];
+			double d;
+			__builtin_memcpy(&d, &val, 8);
+			atomicAdd(&yw[row[s * K + k]], d * xv[s * K + k]);
+		}
+		la += K * lstride;
+	}
+}
+
 //
 // A4L64: the packet's coordinates are exactly 4 bytes (so, from the header's
 // 4-byte alignment, every coordinate is dword-aligned) and it has 64 lanes.
@@ -341,50 +394,20 @@ __device__ __forceinline__ void div_rf1_group(const uint8_t *__restrict__ pk, in
 	// constant strides and the K rows stay in registers between the passes
 	const int nfull = dv / K * K;
 	if (nfull > 0 && lane_id() < L) {
-		int at = crd0_at, la = lens0;
-		for (int p0 = 0; p0 < nfull; p0 += K) {
-			uint32_t row[K];
-			double xv[K];
-#pragma unroll
-			for (int k = 0; k < K; k++) {
-				uint32_t col;
-				if (A4L64) {
-					const uint32_t c = *(const uint32_t *) (pk + at + k * cstride);
-					row[k] = c & ((1u << row_bits) - 1);
-					col = c >> row_bits;
-				} else {
-					col = div_coord<K, C4>(pk, at + k * cstride, cb, cmask, row_bits, row[k]);
-				}
-				xv[k] = x[col_min + col];
+		int at = crd0_at, la = lens0, p0 = 0;
+		if constexpr (K <= 8) {
+			// exactly two full steps (packets of 1024 values in 64 lanes at
+			// K = 8): both steps' gathers are requested before either is
+			// decoded, so the second step's gathers wait behind no decode
+			if (nfull == 2 * K) {
+				div_full_steps<K, 2, C4, A4L64>(pk, at, la, cstride, lstride, cb, cmask, row_bits, col_min,
+						stream_at, x, yw, cursor, val);
+				p0 = nfull;
 			}
-			at += K * cstride;
-			// the step's K length bytes, then its K deltas, are read before its
-			// first accumulation (the scheduling barriers keep them together):
-			// the compiler does not move loads across the LDS atomics, so reads
-			// between the atomics put one LDS round trip per value on the
-			// critical path.
-			int lb[K];
-#pragma unroll
-			for (int k = 0; k < K; k++)
-				lb[k] = pk[la + k * lstride];
-			__builtin_amdgcn_sched_barrier(0); // all length bytes requested together
-			uint64_t dlt[K];
-#pragma unroll
-			for (int k = 0; k < K; k++) {
-				const int len = lb[k] & 15, tz = (lb[k] >> 4) << 2;
-				dlt[k] = lds_le_b32(pk, stream_at + cursor, len) << tz;
-				cursor += len;
-			}
-			__builtin_amdgcn_sched_barrier(0); // then all deltas, before the first accumulation
-#pragma unroll
-			for (int k = 0; k < K; k++) {
-				val += dlt[k];
-				double d;
-				__builtin_memcpy(&d, &val, 8);
-				atomicAdd(&yw[row[k]], d * xv[k]);
-			}
-			la += K * lstride;
 		}
+		for (; p0 < nfull; p0 += K)
+			div_full_steps<K, 1, C4, A4L64>(pk, at, la, cstride, lstride, cb, cmask, row_bits, col_min, stream_at,
+					x, yw, cursor, val);
 	}
 	// the rest (at most K + 1 values per lane), each value tested
 	for (int p0 = nfull; p0 < steps; p0 += K) {
