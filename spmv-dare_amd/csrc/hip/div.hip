@@ -296,17 +296,20 @@ __device__ __forceinline__ uint32_t lds_le_b32x(const uint8_t *base, int pos, ui
 }
 
 // One replication-factor-1 group of a packet staged in LDS (the common case:
-// every value distinct). The loads are unconditional: a step past the lane's
-// last value reads the packet's first coordinate and length byte (positions
-// inside the packet), gathers x[col_min] (a valid column) and decodes a zero
-// delta; only its LDS accumulation is skipped. C4: the coordinates fit 4
-// bytes and are read with one byte-align.
+// every value distinct). C4: the coordinates fit 4 bytes and are read with
+// one byte-align.
 //
 // A step of K values runs in two passes: the first decodes the K columns
 // and requests all K gathers of x (one memory round trip per step), the
-// second re-reads each coordinate and length byte from LDS (cheap) to decode
-// the value and accumulate. Only the K gathered values stay in registers
-// across the passes.
+// second reads the K length bytes and deltas and accumulates. Full steps
+// (div_full_steps) keep the K rows in registers between the passes, and a
+// lane with exactly two full steps requests both steps' gathers at once.
+// The steps past the full ones test each value: their loads are
+// unconditional (a value past the lane's last reads the packet's first
+// coordinate and length byte, positions inside the packet, gathers
+// x[col_min], a valid column, and decodes a zero delta), only the LDS
+// accumulation is skipped, and the second pass re-reads the coordinate from
+// LDS (cheap) so that only the K gathered values stay in registers.
 template <int K, bool C4>
 __device__ __forceinline__ uint32_t div_coord(const uint8_t *pk, int at, int cb, uint32_t cmask, int row_bits,
 		uint32_t &row)
