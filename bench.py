@@ -52,30 +52,62 @@ def alg_bytes(m, n, nnz):
     return 12 * nnz + 4 * (m + 1) + 8 * n + 8 * m
 
 
-REF_CPU_FORMATS = ("csr_naive", "csr_vec", "div_rf")
+REF_CPU_FORMATS = ("csr_naive", "csr_vec", "sell_c_s", "div_rf")
 
 
 def ref_cpu_exe(fmt):
     return os.path.join(ROOT, "oracle", "_ref", "cpu_%s_d.exe" % fmt)
 
 
-def cpu_baseline_reference(line, m, n, nnz, cores, timeout_s):
-    """The reference's own CPU formats (csr_naive, csr_vec, div_rf; compiled
-    from its sources with its own driver, src/bench.cpp + src/bench_spmv.cpp)
-    on the host cores: the driver generates the same matrix from the same
-    parameter line, then times >= 64 iterations and >= 2 s and prints the
-    median (bench_spmv.cpp:313-359). sell_c_s has no x86 kernel in the
-    reference (RISC-V vector intrinsics only), so it is not timed here."""
+def host_cpus():
+    """One logical CPU per physical core among the CPUs this process may run
+    on (sysfs topology), and the CPU model: the 1:1 pinning of the
+    reference's calc_cpu_pinning (config.sh:16-31) with one thread per core."""
+    allowed = sorted(os.sched_getaffinity(0))
+    first = {}
+    for c in allowed:
+        base = "/sys/devices/system/cpu/cpu%d/topology/" % c
+        try:
+            key = (open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip())
+        except OSError:
+            key = ("?", str(c))
+        first.setdefault(key, c)
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return sorted(first.values()), len(allowed), model
+
+
+def cpu_baseline_reference(line, m, n, nnz, timeout_s):
+    """The reference's own CPU formats (csr_naive, csr_vec, div_rf; sell_c_s
+    with its own conversion, analyzer and scheduler around a portable kernel
+    in place of its RISC-V one), each an executable built from its sources
+    with its own driver (src/bench.cpp + src/bench_spmv.cpp). The driver
+    generates the same matrix from the same parameter line, then times >= 64
+    iterations and >= 2 s and prints the median (bench_spmv.cpp:313-359).
+    One OpenMP thread per physical core, pinned with GOMP_CPU_AFFINITY as
+    run.sh:68-72 does."""
     import re
     import subprocess
+    cpus, logical, model = host_cpus()
     env = dict(os.environ, USE_ARTIFICIAL_MATRICES="1", USE_PROCESSES="0", GPU_KERNEL="0",
-               USE_RCM_REORDERING="0", COOLDOWN="0", NUM_PROCESSES="1", OMP_NUM_THREADS=str(cores),
+               USE_RCM_REORDERING="0", COOLDOWN="0", NUM_PROCESSES="1", OMP_NUM_THREADS=str(len(cpus)),
+               GOMP_CPU_AFFINITY=",".join(str(c) for c in cpus), OMP_DYNAMIC="false",
                CSRCV_NUM_PACKET_VALS="16384", DIV_VC_TOLERANCE="0", DIV_PACKET_REORDERING="0",
                DIV_STEALING_ENABLED="1")
+    env.pop("OMP_PROC_BIND", None)
+    env.pop("OMP_PLACES", None)
     B = alg_bytes(m, n, nnz)
     per = {}
     for fmt in REF_CPU_FORMATS:
         exe = ref_cpu_exe(fmt)
+        if not os.path.exists(exe):
+            continue
         env["PROGG"] = fmt
         t0 = time.perf_counter()
         try:
@@ -90,43 +122,69 @@ def cpu_baseline_reference(line, m, n, nnz, cores, timeout_s):
             continue
         med = float(mt.group(2))
         per[fmt] = {"gbps": B / med / 1e9, "gflops": 2 * nnz / med / 1e9, "median_ms": med * 1e3,
-                    "loops": int(ml.group(1)) if ml else None, "wall_s": time.perf_counter() - t0}
+                    "min_ms": float(mt.group(1)) * 1e3, "loops": int(ml.group(1)) if ml else None,
+                    "wall_s": time.perf_counter() - t0,
+                    "kernel": "reference" if fmt != "sell_c_s" else "reference conversion and scheduler, "
+                                                                    "portable lane-loop kernel"}
         log("cpu baseline %-9s %7.1f GB/s  median %.2f ms  (%d loops, %.0f s with generation)" % (
             fmt, per[fmt]["gbps"], med * 1e3, per[fmt]["loops"] or 0, per[fmt]["wall_s"]))
     if not per:
         return None
     best = max(per, key=lambda k: per[k]["gbps"])
-    return {"value": round(per[best]["gbps"], 2), "unit": "GB/s", "cores": cores, "kind": "reference",
+    return {"value": round(per[best]["gbps"], 2), "unit": "GB/s", "cores": len(cpus), "kind": "reference",
             "sample": "reference CPU %s (best of %s; the reference's own driver, >= 64 iterations and >= 2 s, "
-                      "median) on the full matrix, OpenMP on %d host threads, AVX2 build" % (
-                          best, ", ".join(sorted(per)), cores),
-            "format": best, "gflops": round(per[best]["gflops"], 2),
+                      "median) on the full matrix, one OpenMP thread per physical core pinned 1:1 "
+                      "(%d cores of %d logical CPUs, %s), AVX2 build" % (
+                          best, ", ".join(sorted(per)), len(cpus), logical, model),
+            "format": best, "gflops": round(per[best]["gflops"], 2), "cpu_model": model,
+            "physical_cores": len(cpus), "logical_cpus": logical,
             "formats": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                         for k, v in per.items()}}
 
 
-def cpu_baseline_port(a, x, y_gpu, budget_s, cores):
+def cpu_baseline_port(a, x, budget_s):
     """The oracle's OpenMP csr_naive (equal-row split as csr.cpp:74) on the
     host cores, on the same matrix: one warm-up call, then iterations until
     the time budget (at least 3), median time. Used where the reference's
     CPU executables were not built."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle
-    y, t = pyoracle.csr_naive_omp(a.row_ptr, a.col_ind, a.values, x, threads=cores)
+    cpus, logical, model = host_cpus()
+    y, t = pyoracle.csr_naive_omp(a.row_ptr, a.col_ind, a.values, x, threads=len(cpus))
     times = []
     t_end = time.perf_counter() + budget_s
     while len(times) < 3 or (time.perf_counter() < t_end and len(times) < 128):
         t0 = time.perf_counter()
-        pyoracle.csr_naive_omp(a.row_ptr, a.col_ind, a.values, x, threads=cores, y=y)
+        pyoracle.csr_naive_omp(a.row_ptr, a.col_ind, a.values, x, threads=len(cpus), y=y)
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
-    err = pyoracle.normwise_rel_err(y_gpu, y) if y_gpu is not None else None
-    return {"value": alg_bytes(a.m, a.n, a.nnz) / med / 1e9, "unit": "GB/s", "cores": t,
+    return {"value": round(alg_bytes(a.m, a.n, a.nnz) / med / 1e9, 2), "unit": "GB/s", "cores": t,
             "kind": "port",
             "sample": "oracle csr_naive (OpenMP, equal-row split) on the full matrix, "
-                      "%d timed iterations after 1 warm-up, median %.2f ms" % (len(times), med * 1e3),
-            "gflops": 2 * a.nnz / med / 1e9}, err
+                      "%d timed iterations after 1 warm-up, median %.2f ms (%s)" % (len(times), med * 1e3, model),
+            "gflops": 2 * a.nnz / med / 1e9, "cpu_model": model}
+
+
+def check_leg(a, x, ys, best_key):
+    """The oracle as the checker: y of the reference's csr_naive semantics
+    (sequential left-to-right row sums; OpenMP over rows, which does not change
+    any row's sum) and the double-double oracle, against every GPU y.
+    Returns {format key: normwise relative error vs csr_naive} and the best
+    format's error against the double-double y."""
+    import numpy as np
+    import pyoracle
+    cpus, _, _ = host_cpus()
+    t0 = time.perf_counter()
+    y_ref, _ = pyoracle.csr_naive_omp(a.row_ptr, a.col_ind, a.values, x, threads=len(cpus))
+    nrm = float(np.linalg.norm(y_ref))
+    errs = {k: float(np.linalg.norm(y - y_ref) / nrm) for k, y in ys.items()}
+    hi, lo = pyoracle.csr_dd(a.row_ptr, a.col_ind, a.values, x)
+    dd = float(np.linalg.norm((ys[best_key] - hi) - lo) / np.linalg.norm(hi))
+    dd_naive = float(np.linalg.norm((y_ref - hi) - lo) / np.linalg.norm(hi))
+    log("check: %d formats against the oracle in %.1f s; worst normwise error %.2e; best format vs "
+        "double-double %.2e (csr_naive itself %.2e)" % (len(errs), time.perf_counter() - t0, max(errs.values()),
+                                                         dd, dd_naive))
+    return errs, dd, dd_naive
 
 
 PROFILED = {  # the variant each profile summary's byte counters were taken on
@@ -198,8 +256,8 @@ def run_e2e(args, gen, hip, np):
     from spmvdare import tuning
     model = os.path.join(ROOT, "spmv-dare_amd", "spmvdare", "models", "selector.json")
     sel = tuning.Selector.load(model)
-    allowed = np.array([not (k == "csr_vec" and kw.get("block_nnz", 256) > 256) for k, kw in tuning.VARIANTS])
-    csr = tuning.VARIANTS.index(("csr_vec", {"block_nnz": 256, "lanes_per_row": 0}))
+    allowed = tuning.batch_allowed()
+    csr = tuning.DEFAULT_CSR
     count = args.e2e_count
     params = tuning.sample_params(55, count * 2, **E2E_RANGES)
     chunk = max(1, args.e2e_chunk)
@@ -299,23 +357,22 @@ def run_e2e(args, gen, hip, np):
     print(json.dumps(out), flush=True)
 
 
-def run_batched(args, gen, hip, np):
+def batched_leg(args, gen, hip, np, count, steps, warmup):
     """Batched workload: `count` generated matrices of uneven sizes (rows =
     cols log-uniform in [1e3, 2e5], avg 4..64, ...; master seed 21), each with
     its own x, computed by one persistent work-queue launch replayed from a
     hipGraph, with every matrix's format chosen by the learned selector;
-    compared on the same GPU with plain per-matrix binned-CSR launches.
-    Prints one JSON line (matrices/s)."""
+    compared on the same GPU with plain per-matrix binned-CSR launches (one
+    launch per matrix, back to back) and with the same persistent launch over
+    all-CSR members. Every member's batched y is checked against the device
+    csr_naive of that matrix (a sequential left-to-right sum per row, the
+    oracle's csr_naive bit for bit). Returns a dict."""
     from spmvdare import tuning
-    count = args.count
     model = os.path.join(ROOT, "spmv-dare_amd", "spmvdare", "models", "selector.json")
     sel = tuning.Selector.load(model) if os.path.exists(model) and not args.no_selector else None
     fixed = tuning.VARIANTS.index(("sell_c_s", {"sigma": 256, "lanes_per_row": 4, "unroll": 4}))
-    csr = tuning.VARIANTS.index(("csr_vec", {"block_nnz": 256, "lanes_per_row": 0}))  # the default csr_vec
-    # the persistent launch stages CSR row blocks of at most 256 nnz at full
-    # occupancy (a 512-nnz member would halve it for every member): the
-    # selector chooses among the variants that keep it
-    allowed = np.array([not (k == "csr_vec" and kw.get("block_nnz", 256) > 256) for k, kw in tuning.VARIANTS])
+    csr = tuning.DEFAULT_CSR
+    allowed = tuning.batch_allowed()
     params = tuning.sample_params(21, count * 2)
     t0 = time.perf_counter()
     members, total_nnz, total_bytes, chosen = [], 0, 0, []
@@ -340,10 +397,11 @@ def run_batched(args, gen, hip, np):
         total_nnz += a.nnz
         total_bytes += alg_bytes(a.m, a.n, a.nnz)
         dm.a = None  # host arrays no longer needed
-        if len(members) % 256 == 0:
+        if len(members) % 512 == 0:
             log("batched: %d/%d matrices prepared, %.0f s" % (len(members), count, time.perf_counter() - t0))
+    prep_s = time.perf_counter() - t0
     log("batched: %d matrices, %d nnz, prepared in %.0f s; selector=%s" % (
-        len(members), total_nnz, time.perf_counter() - t0, "mlp" if sel is not None else "fixed"))
+        len(members), total_nnz, prep_s, "mlp" if sel is not None else "fixed"))
     batch = hip.Batch(args.tile)
     batch.set_queues(args.queues)
     base = hip.Batch(args.tile)
@@ -353,47 +411,70 @@ def run_batched(args, gen, hip, np):
     batch.finalize(use_graph=True)
     base.finalize(use_graph=False)
     hip.synchronize()
-    ms_batch = batch.time(args.warmup, args.steps)
-    ms_plain = base.time(max(1, args.warmup // 2), max(3, args.steps // 4), per_matrix=True)
+    ms_batch = batch.time(warmup, steps)
+    ms_plain = base.time(max(1, warmup // 2), max(3, steps // 4), per_matrix=True)
     # the same persistent launch with every member as plain binned CSR: the
     # batching gain without the selector
-    ms_base = base.time(max(1, args.warmup // 2), max(3, args.steps // 4))
-    # consistency: every member's batched y against its own per-matrix CSR y
-    worst = 0.0
+    ms_base = base.time(max(1, warmup // 2), max(3, steps // 4))
+    batch.run()
+    batch.synchronize()
+    # every member's batched y against its csr_naive y on the device
+    t1 = time.perf_counter()
+    worst, worst_csr = 0.0, 0.0
     for dm, f_sel, f_csr, dy_csr in members:
         yb, yc = dm.dy.download(), dy_csr.download()
-        nrm = np.linalg.norm(yc)
-        worst = max(worst, float(np.linalg.norm(yb - yc) / nrm) if nrm > 0 else float(np.linalg.norm(yb)))
-    t_b, t_p = float(np.median(ms_batch)), float(np.median(ms_plain))
+        f_ref = hip.csr_to_format_device(dm.rp, dm.col, dm.val, dm.m, dm.n, dm.nnz, kind="csr_naive")
+        f_ref.spmv_device(dm.dx, dy_csr)
+        hip.synchronize()
+        yr = dy_csr.download()
+        f_ref.destroy()
+        nrm = np.linalg.norm(yr)
+        rel = (lambda y_: float(np.linalg.norm(y_ - yr) / nrm) if nrm > 0 else float(np.linalg.norm(y_)))
+        worst, worst_csr = max(worst, rel(yb)), max(worst_csr, rel(yc))
+    check_s = time.perf_counter() - t1
+    t_b, t_p, t_a = float(np.median(ms_batch)), float(np.median(ms_plain)), float(np.median(ms_base))
     hist = np.bincount(np.array(chosen), minlength=len(tuning.VARIANTS))
     out = {
-        "metric": "batched FP64 SpMV matrices/s (selector-chosen formats, one persistent hipGraph launch)",
-        "value": round(len(members) / (t_b * 1e-3), 1), "unit": "matrices/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_b, 4), "higher_is_better": True,
-        "dtype": "f64", "data": "synthetic (reference generator restated bit-exactly)",
-        "config": {"workload": "%d generated matrices, master seed 21" % len(members), "nnz": total_nnz,
-                   "alg_bytes": total_bytes, "tile_units": args.tile,
-                   "selector": "mlp" if sel is not None else tuning.variant_name(fixed)},
-        "batched_gbps": round(total_bytes / (t_b * 1e-3) / 1e9, 1),
+        "workload": "%d generated matrices, master seed 21 (rows 1e3..2e5, avg 4..64 nnz/row)" % len(members),
+        "matrices": len(members), "nnz": total_nnz, "alg_bytes": total_bytes,
+        "selector": "mlp" if sel is not None else tuning.variant_name(fixed),
+        "steps": steps, "ms": round(t_b, 4), "min_ms": round(float(np.min(ms_batch)), 4),
+        "matrices_per_s": round(len(members) / (t_b * 1e-3), 1),
+        "gbps": round(total_bytes / (t_b * 1e-3) / 1e9, 1),
         "plain_per_matrix_csr": {"ms": round(t_p, 4), "matrices_per_s": round(len(members) / (t_p * 1e-3), 1),
                                  "gbps": round(total_bytes / (t_p * 1e-3) / 1e9, 1)},
         "speedup_vs_plain_csr": round(t_p / t_b, 3),
-        "batched_all_csr": {"ms": round(float(np.median(ms_base)), 4),
-                            "speedup_vs_plain_csr": round(t_p / float(np.median(ms_base)), 3)},
+        "batched_all_csr": {"ms": round(t_a, 4), "speedup_vs_plain_csr": round(t_p / t_a, 3)},
+        "max_normwise_err_vs_csr_naive": worst,
+        "max_normwise_err_vs_csr_naive_plain_csr": worst_csr,
+        "prep_s": round(prep_s, 1), "check_s": round(check_s, 1),
         "batch_min_block_nnz": args.batch_bnnz,
-        "max_normwise_diff_vs_csr": worst,
         "chosen_variants": {tuning.variant_name(i): int(c) for i, c in enumerate(hist) if c},
     }
     log("batched %.3f ms (%.0f matrices/s, %.0f GB/s) vs plain per-matrix CSR %.3f ms: x%.2f; "
-        "batched all-CSR %.3f ms" % (t_b, out["value"], out["batched_gbps"], t_p, t_p / t_b,
-                                     out["batched_all_csr"]["ms"]))
+        "batched all-CSR %.3f ms; max err vs csr_naive %.2e" % (
+            t_b, out["matrices_per_s"], out["gbps"], t_p, t_p / t_b, t_a, worst))
+    batch.destroy()
+    base.destroy()
+    return out
+
+
+def run_batched(args, gen, hip, np):
+    """--batched: the batched leg alone, as its own JSON line (matrices/s)."""
+    b = batched_leg(args, gen, hip, np, args.count, args.steps, args.warmup)
+    out = {"metric": "batched FP64 SpMV matrices/s (selector-chosen formats, one persistent hipGraph launch)",
+           "value": b["matrices_per_s"], "unit": "matrices/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": b["ms"], "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic (reference generator restated bit-exactly)",
+           "config": {"workload": b["workload"]}, "batched": b}
     print(json.dumps(out), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--probe", action="store_true", help="measure stream read / copy bandwidth only")
-    ap.add_argument("--batched", action="store_true", help="batched workload instead of one matrix")
+    ap.add_argument("--batched", action="store_true", help="batched workload only (its own JSON line)")
+    ap.add_argument("--no-batched", action="store_true", help="leave the batched leg out of the default run")
     ap.add_argument("--count", type=int, default=2048, help="matrices of the batched workload")
     ap.add_argument("--tile", type=int, default=0,
                     help="batched launch: consecutive units per ticket (0 = the library default)")
@@ -415,7 +496,7 @@ def main():
                     help="comma list of kind[:sigma], or 'all'")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of the port's timing loop")
     ap.add_argument("--cpu-timeout", type=int, default=150, help="seconds per reference CPU format run")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="no CPU baseline and no oracle check of y")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -445,6 +526,7 @@ def main():
     log("generated %s: m=%d nnz=%d in %.1f s" % (args.workload, a.m, a.nnz, time.perf_counter() - t0))
     x = gen.make_x(a.n, p.seed + 1)
     B = alg_bytes(a.m, a.n, a.nnz)
+    m, n, nnz = a.m, a.n, a.nnz
 
     cands = CANDIDATES
     if args.formats == "sweep":
@@ -455,7 +537,7 @@ def main():
 
     dx = hip.DeviceArray.from_host(x)
     dy = hip.DeviceArray(a.m)
-    results = []
+    results, ys = [], {}
     best = None
     for kind, kw in cands:
         t0 = time.perf_counter()
@@ -464,42 +546,47 @@ def main():
         hip.synchronize()
         t_conv = time.perf_counter() - t0
         steps = args.steps if kind != "csr_naive" else min(args.steps, 16)
-        hip.synchronize()
+        dy.fill_bytes(0xff)  # NaN: a row the kernel does not write shows up in the check
         ms = f.time(dx, dy, args.warmup, steps)
         hip.synchronize()
-        mean_ms = float(ms.mean())
-        r = {"format": kind, "params": kw, "mean_ms": mean_ms, "median_ms": float(np.median(ms)),
-             "min_ms": float(ms.min()), "gbps": B / (mean_ms * 1e-3) / 1e9,
-             "gflops": 2 * a.nnz / (mean_ms * 1e-3) / 1e9, "conv_ms_device": info["conv_ms"],
+        med = float(np.median(ms))
+        r = {"format": kind, "params": kw, "median_ms": med, "min_ms": float(ms.min()),
+             "mean_ms": float(ms.mean()), "gbps": B / (med * 1e-3) / 1e9,
+             "gflops": 2 * a.nnz / (med * 1e-3) / 1e9, "conv_ms_device": info["conv_ms"],
              "csr_to_format_s": t_conv, "padded_nnz": info["padded_nnz"],
              "mem_footprint": info["mem_footprint"], "steps": steps}
+        key = "%s %s" % (kind, json.dumps(kw, sort_keys=True))
+        r["key"] = key
+        ys[key] = dy.download()
         results.append(r)
-        log("%-10s %-16s %8.4f ms  %7.1f GB/s  %5.1f%% peak  conv %.1f ms" % (
-            kind, kw, mean_ms, r["gbps"], 100 * r["gbps"] / HBM_PEAK_GBS, info["conv_ms"]))
-        if best is None or r["gbps"] > best[0]["gbps"]:
-            if best is not None:
-                best[1].destroy()
-            best = (r, f)
-        else:
-            f.destroy()
-
-    r, f = best
-    # y of the winning format for the CPU-baseline cross-check
-    f.spmv_device(dx, dy)
-    hip.synchronize()
-    y_gpu = dy.download()
-
+        log("%-10s %-16s median %8.4f ms (min %.4f, mean %.4f)  %7.1f GB/s  %5.1f%% peak  conv %.1f ms" % (
+            kind, kw, med, r["min_ms"], r["mean_ms"], r["gbps"], 100 * r["gbps"] / HBM_PEAK_GBS, info["conv_ms"]))
+        if best is None or r["gbps"] > best["gbps"]:
+            best = r
+        f.destroy()
+    r = best
     traffic = profiled_traffic(args.workload, r["format"], r["params"])
 
-    cpu = None
+    # cpu leg: the oracle as the checker of every y, then the CPU baseline
+    cpu, errs, dd, dd_naive = None, None, None, None
     if not args.no_cpu:
-        cores = len(os.sched_getaffinity(0))
-        log("cpu baseline on %d host threads ..." % cores)
-        if all(os.path.exists(ref_cpu_exe(f)) for f in REF_CPU_FORMATS):
-            cpu = cpu_baseline_reference(line, a.m, a.n, a.nnz, cores, args.cpu_timeout)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        errs, dd, dd_naive = check_leg(a, x, ys, r["key"])
+        for res in results:
+            res["normwise_err_vs_csr_naive"] = errs[res["key"]]
+        ys.clear()
+        cpus, _, _ = host_cpus()
+        log("cpu baseline on %d physical cores ..." % len(cpus))
+        cpu = cpu_baseline_reference(line, a.m, a.n, a.nnz, args.cpu_timeout)
         if cpu is None:
-            cpu, err = cpu_baseline_port(a, x, y_gpu, args.cpu_budget, cores)
-            log("cpu baseline (port) %.1f GB/s; GPU y vs CPU y normwise %.2e" % (cpu["value"], err))
+            cpu = cpu_baseline_port(a, x, args.cpu_budget)
+            log("cpu baseline (port) %.1f GB/s" % cpu["value"])
+    ys.clear()
+
+    batched = None
+    if not args.no_batched:
+        del a
+        batched = batched_leg(args, gen, hip, np, args.count, args.steps, args.warmup)
 
     out = {
         "metric": "FP64 SpMV GB/s (% HBM peak); learned-selected batched vs plain CSR",
@@ -508,22 +595,28 @@ def main():
         "n_gpus": 1,
         "steps": r["steps"],
         "warmup": args.warmup,
-        "ms_per_step": round(r["mean_ms"], 6),
+        "ms_per_step": round(r["median_ms"], 6),
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: reference generator restated bit-exactly; values 2v-1, x from seed+1",
-        "config": {"workload": desc, "m": a.m, "n": a.n, "nnz": a.nnz, "format": r["format"],
-                   "params": r["params"], "pct_hbm_peak": round(100 * r["gbps"] / HBM_PEAK_GBS, 2),
-                   "gflops": round(r["gflops"], 2), "alg_bytes_per_step": B,
-                   "formats": [{k: v for k, v in x_.items() if k in ("format", "params", "mean_ms", "gbps")}
+        "config": {"workload": desc, "m": m, "n": n, "nnz": nnz,
+                   "format": r["format"], "params": r["params"],
+                   "pct_hbm_peak": round(100 * r["gbps"] / HBM_PEAK_GBS, 2),
+                   "gflops": round(r["gflops"], 2), "alg_bytes_per_step": B, "timing": "median of the timed steps",
+                   "min_ms": round(r["min_ms"], 6), "mean_ms": round(r["mean_ms"], 6),
+                   "formats": [{k: v for k, v in x_.items()
+                                if k in ("format", "params", "median_ms", "min_ms", "gbps", "normwise_err_vs_csr_naive")}
                                for x_ in results]},
-        "roofline": {"bound": "hbm", "achieved": round(B / (r["mean_ms"] * 1e-3) / 1e9, 2),
+        "normwise_err_vs_csr_naive": errs[r["key"]] if errs else None,
+        "normwise_err_vs_double_double": dd,
+        "roofline": {"bound": "hbm", "achieved": round(B / (r["median_ms"] * 1e-3) / 1e9, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(B / (r["mean_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "frac": round(B / (r["median_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

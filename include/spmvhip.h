@@ -91,7 +91,11 @@ typedef struct spmvhip_params {
 	                         coalesced permutation pass y[r] = ys[inv[r]] (SELL_C_S) */
 	int div_acc_copies;   /* DIV_RF: copies of the LDS row accumulators the lanes add into (a power
 	                         of two, at most 64; 0 = up to 4 within 4 KiB for the largest packet) */
-	int reserved[5];
+	int narrow_loads;     /* CSR_VEC: 1 = one stored entry per lane and load (4-byte column, 8-byte
+	                         value loads); 0 (default) = two adjacent entries per lane and load
+	                         (8-byte column, 16-byte value loads: a third fewer vector-memory
+	                         instructions per nonzero) */
+	int reserved[4];
 } spmvhip_params;
 
 typedef struct spmvhip_format spmvhip_format;   /* opaque */

@@ -359,4 +359,167 @@ int encode(const int32_t *rp, const int32_t *col, const double *val, long m, lon
 	return 0;
 }
 
+namespace {
+
+uint32_t get32(const uint8_t *p)
+{
+	uint32_t v;
+	memcpy(&v, p, 4);
+	return v;
+}
+
+uint64_t get_le(const uint8_t *p, int nbytes)
+{
+	uint64_t v = 0;
+	for (int b = 0; b < nbytes; b++)
+		v |= (uint64_t) p[b] << (8 * b);
+	return v;
+}
+
+struct SlotSizes {
+	size_t runs_at, bits_at, coords_at, lens_at, streams_at, total;
+};
+
+SlotSizes slot_sizes(int L, uint32_t nv, uint32_t nu, uint32_t nrfs, uint32_t nruns, uint32_t nslots, int cb,
+		size_t stream_bytes)
+{
+	SlotSizes z;
+	z.runs_at = (size_t) slot_header_bytes(L) + 8 * (size_t) nrfs;
+	z.bits_at = (z.runs_at + 4 * (size_t) nruns + 7) / 8 * 8;
+	z.coords_at = z.bits_at + 8 * (((size_t) nslots + 63) / 64);
+	z.lens_at = z.coords_at + (size_t) nv * cb;
+	z.streams_at = z.lens_at + nu;
+	z.total = z.streams_at + stream_bytes;
+	return z;
+}
+
+} // namespace
+
+int to_slot_layout(const Stream &st, SlotLayout *out, const char **err)
+{
+	const long np = st.npackets();
+	const int L = st.L;
+	std::vector<std::vector<uint8_t>> pk(np);
+	int fail = 0;
+	uint32_t max_slots = 0, max_rows = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(max : max_slots, max_rows)
+	for (long p = 0; p < np; p++) {
+		const uint8_t *src = &st.bytes[st.pk_off[p]];
+		const uint32_t nv = get32(src), nu = get32(src + 4), nrows = get32(src + 8), row_min = get32(src + 12);
+		const uint32_t col_min = get32(src + 16);
+		const int row_bits = src[20], col_bits = src[21];
+		const uint32_t nrfs = get32(src + 24);
+		const int cb = (row_bits + col_bits + 7) >> 3;
+		const size_t hb = (size_t) header_bytes(L);
+		const uint8_t *coords = src + hb + 8 * (size_t) nrfs;
+		const uint64_t rmask = row_bits ? (1ull << row_bits) - 1 : 0;
+		std::vector<uint32_t> rows(nv), cols(nv);
+		for (uint32_t k = 0; k < nv; k++) {
+			const uint64_t c = get_le(coords + (size_t) k * cb, cb);
+			rows[k] = (uint32_t) (c & rmask);
+			cols[k] = col_min + (uint32_t) (c >> row_bits);
+		}
+		std::vector<uint32_t> ucol(cols);
+		std::sort(ucol.begin(), ucol.end());
+		ucol.erase(std::unique(ucol.begin(), ucol.end()), ucol.end());
+		const uint32_t nslots = (uint32_t) ucol.size();
+		std::vector<uint32_t> run_col;
+		std::vector<uint64_t> bits((nslots + 63) / 64, 0);
+		for (uint32_t s = 0; s < nslots; s++) {
+			if (s == 0 || ucol[s] != ucol[s - 1] + 1) {
+				run_col.push_back(ucol[s]);
+				bits[s >> 6] |= 1ull << (s & 63);
+			}
+		}
+		const int rb = (int) req_bits(nrows > 0 ? nrows - 1 : 0);
+		const int sb = (int) req_bits(nslots > 0 ? nslots - 1 : 0);
+		if (rb + sb > 32) {
+#pragma omp critical
+			{
+				fail = 1;
+				*err = "x-slot coordinate above 32 bits";
+			}
+			continue;
+		}
+		const int ncb = std::max(1, (rb + sb + 7) >> 3);
+		size_t stream_bytes = 0;
+		for (int l = 0; l < L; l++)
+			stream_bytes += get32(src + kHeaderFixed + 4 * l);
+		const SlotSizes z = slot_sizes(L, nv, nu, nrfs, (uint32_t) run_col.size(), nslots, ncb, stream_bytes);
+		std::vector<uint8_t> &o = pk[p];
+		o.assign((z.total + 15) / 16 * 16, 0);
+		put32(o, 0, nv);
+		put32(o, 4, nu);
+		put32(o, 8, nrows);
+		put32(o, 12, row_min);
+		put32(o, 16, nslots);
+		o[20] = (uint8_t) rb;
+		o[21] = (uint8_t) sb;
+		o[22] = (uint8_t) ncb;
+		put32(o, 24, nrfs);
+		put32(o, 28, (uint32_t) run_col.size());
+		memcpy(&o[kSlotHeaderFixed], src + kHeaderFixed, 4 * (size_t) L);            // lane sizes
+		memcpy(&o[slot_header_bytes(L)], src + hb, 8 * (size_t) nrfs);                // rfs, unique_per_rf
+		for (size_t r = 0; r < run_col.size(); r++)
+			put32(o, z.runs_at + 4 * r, run_col[r]);
+		if (!bits.empty())
+			memcpy(&o[z.bits_at], bits.data(), 8 * bits.size());
+		for (uint32_t k = 0; k < nv; k++) {
+			const uint32_t slot = (uint32_t) (std::lower_bound(ucol.begin(), ucol.end(), cols[k]) - ucol.begin());
+			const uint64_t c = (uint64_t) rows[k] | ((uint64_t) slot << rb);
+			for (int b = 0; b < ncb; b++)
+				o[z.coords_at + (size_t) k * ncb + b] = (uint8_t) (c >> (8 * b));
+		}
+		const uint8_t *lens = coords + (size_t) nv * cb;
+		memcpy(&o[z.lens_at], lens, nu + stream_bytes); // val_lens and lane streams, unchanged
+		max_slots = std::max(max_slots, nslots);
+		max_rows = std::max(max_rows, nrows);
+	}
+	if (fail)
+		return -1;
+	out->pk_off.assign(np + 1, 0);
+	int64_t at = 0, mx = 0;
+	for (long p = 0; p < np; p++) {
+		out->pk_off[p] = at;
+		at += (int64_t) pk[p].size();
+		mx = std::max<int64_t>(mx, (int64_t) pk[p].size());
+	}
+	out->pk_off[np] = at;
+	out->bytes.assign((size_t) at + 16, 0); // + 16: the decoder reads whole aligned words
+#pragma omp parallel for schedule(static)
+	for (long p = 0; p < np; p++)
+		if (!pk[p].empty())
+			memcpy(&out->bytes[out->pk_off[p]], pk[p].data(), pk[p].size());
+	out->max_packet = mx;
+	out->max_slots = max_slots;
+	out->max_rows = max_rows;
+	return 0;
+}
+
+void slot_packet_coords(const uint8_t *pk, int L, int32_t *rows, int32_t *cols)
+{
+	const uint32_t nv = get32(pk), nu = get32(pk + 4), row_min = get32(pk + 12), nslots = get32(pk + 16);
+	const int rb = pk[20], cb = pk[22];
+	const uint32_t nrfs = get32(pk + 24), nruns = get32(pk + 28);
+	const SlotSizes z = slot_sizes(L, nv, nu, nrfs, nruns, nslots, cb, 0);
+	std::vector<uint32_t> slot_col(nslots);
+	long r = -1;
+	uint32_t start = 0;
+	for (uint32_t s = 0; s < nslots; s++) {
+		uint64_t w;
+		memcpy(&w, pk + z.bits_at + 8 * (s >> 6), 8);
+		if (w >> (s & 63) & 1) {
+			r++;
+			start = s;
+		}
+		slot_col[s] = get32(pk + z.runs_at + 4 * r) + (s - start);
+	}
+	const uint64_t rmask = rb ? (1ull << rb) - 1 : 0;
+	for (uint32_t k = 0; k < nv; k++) {
+		const uint64_t c = get_le(pk + z.coords_at + (size_t) k * cb, cb);
+		rows[k] = (int32_t) (row_min + (uint32_t) (c & rmask));
+		cols[k] = (int32_t) slot_col[c >> rb];
+	}
+}
+
 } // namespace spmvdiv

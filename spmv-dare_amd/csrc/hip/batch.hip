@@ -137,7 +137,7 @@ __device__ __forceinline__ void batch_unit(const BatchMat *__restrict__ mats, co
 		csr_block_at<BNNZ>(global_ptr<const int32_t>(M.rp), col, val, x, y, r.r0, r.r1, r.j0, r.j1, M.split_nnz,
 				M.partials != 0, M.lanes, prod);
 	} else if (kind == kUnitSeg) {
-		const double sum = wave_row_sum(col, val, x, r.j0, r.j1);
+		const double sum = wave_row_sum_pairs(col, val, x, r.j0, r.j1);
 		if (lane_id() == 0)
 			global_ptr<double>(M.partials)[r.u] = sum;
 	} else {

@@ -217,6 +217,19 @@ int spmvhip_synchronize(void)
 	return SPMVHIP_OK;
 }
 
+// Device CSR arrays of a handle. col and val carry a zeroed tail of kCsrPad
+// entries: the paired CSR stream loads read entries [j, j + 1] with j even,
+// so the last pair of a matrix with an odd nnz reaches one entry past it.
+static int alloc_csr(spmvhip_format *f)
+{
+	SPMVHIP_TRY(f->row_ptr.alloc((size_t) (f->m + 1) * sizeof(int32_t)));
+	SPMVHIP_TRY(f->col.alloc((size_t) (f->nnz + kCsrPad) * sizeof(int32_t)));
+	SPMVHIP_TRY(f->val.alloc((size_t) (f->nnz + kCsrPad) * sizeof(double)));
+	SPMVHIP_CHECK(hipMemsetAsync(f->col.as<int32_t>() + f->nnz, 0, kCsrPad * sizeof(int32_t), f->stream));
+	SPMVHIP_CHECK(hipMemsetAsync(f->val.as<double>() + f->nnz, 0, kCsrPad * sizeof(double), f->stream));
+	return SPMVHIP_OK;
+}
+
 int spmvhip_csr_to_format(const int32_t *row_ptr, const int32_t *col_ind, const double *values, long m, long n,
 		long nnz, long symmetric, long symmetry_expanded, const spmvhip_params *params, spmvhip_format **out)
 {
@@ -230,11 +243,7 @@ int spmvhip_csr_to_format(const int32_t *row_ptr, const int32_t *col_ind, const 
 	SPMVHIP_TRY(validate_csr(row_ptr, col_ind, m, n, nnz));
 	spmvhip_format *f = nullptr;
 	SPMVHIP_TRY(create_handle(m, n, nnz, params, &f));
-	int rc = f->row_ptr.alloc((size_t) (m + 1) * sizeof(int32_t));
-	if (!rc)
-		rc = f->col.alloc((size_t) nnz * sizeof(int32_t));
-	if (!rc)
-		rc = f->val.alloc((size_t) nnz * sizeof(double));
+	int rc = alloc_csr(f);
 	if (!rc) {
 		hipError_t e = hipMemcpyAsync(f->row_ptr.p, row_ptr, (m + 1) * sizeof(int32_t), hipMemcpyHostToDevice, f->stream);
 		if (e == hipSuccess && nnz)
@@ -274,11 +283,7 @@ int spmvhip_csr_to_format_device(const int32_t *d_row_ptr, const int32_t *d_col_
 	}
 	spmvhip_format *f = nullptr;
 	SPMVHIP_TRY(create_handle(m, n, nnz, params, &f));
-	int rc = f->row_ptr.alloc((size_t) (m + 1) * sizeof(int32_t));
-	if (!rc)
-		rc = f->col.alloc((size_t) nnz * sizeof(int32_t));
-	if (!rc)
-		rc = f->val.alloc((size_t) nnz * sizeof(double));
+	int rc = alloc_csr(f);
 	if (!rc) {
 		hipError_t e = hipMemcpyAsync(f->row_ptr.p, d_row_ptr, (m + 1) * sizeof(int32_t), hipMemcpyDeviceToDevice,
 				f->stream);

@@ -14,6 +14,9 @@ namespace spmvhip {
 constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kBlock = 256;        // threads per workgroup (4 waves)
 constexpr int kWavesPerBlock = kBlock / kWave;
+// zeroed entries after the last stored column index / value of the device
+// CSR arrays (the paired stream loads read up to one entry past nnz)
+constexpr int kCsrPad = 16;
 
 // Records a message for spmvhip_last_error() and returns `code`.
 int set_error(int code, const char *fmt, ...);

@@ -198,7 +198,7 @@ int csr_vec_convert(spmvhip_format *f)
 
 // ------------------------------------------------------------------ binned kernels
 
-template <int BNNZ_MAX>
+template <int BNNZ_MAX, bool NARROW>
 __global__ void __launch_bounds__(kBlock) csr_binned_kernel(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
 		const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
 		const int32_t *__restrict__ blocks, long nblocks, int split_nnz, int lanes_cfg, int remap)
@@ -208,7 +208,8 @@ __global__ void __launch_bounds__(kBlock) csr_binned_kernel(const int32_t *__res
 	const long w = xcd_logical_block(remap) * kWavesPerBlock + wv;
 	if (w >= nblocks)
 		return; // waves never synchronise with each other: early exit is safe
-	csr_block<BNNZ_MAX>(rp, col, val, x, y, blocks[w], blocks[w + 1], split_nnz, true, lanes_cfg, prod[wv]);
+	csr_block<BNNZ_MAX, NARROW>(rp, col, val, x, y, blocks[w], blocks[w + 1], split_nnz, true, lanes_cfg,
+			prod[wv]);
 }
 
 // Persistent variant: a grid of resident workgroups whose waves take blocks
@@ -247,6 +248,7 @@ csr_binned_pipe_kernel(const int32_t *__restrict__ rp,
 	}
 }
 
+template <bool NARROW>
 __global__ void __launch_bounds__(kBlock) csr_split_kernel(const int32_t *__restrict__ col, const double *__restrict__ val,
 		const double *__restrict__ x, const int32_t *__restrict__ desc, long nsegs, double *__restrict__ partials)
 {
@@ -254,7 +256,8 @@ __global__ void __launch_bounds__(kBlock) csr_split_kernel(const int32_t *__rest
 	const long w = (long) blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
 	if (w >= nsegs)
 		return;
-	const double sum = wave_row_sum(col, val, x, desc[3 * w + 1], desc[3 * w + 2]);
+	const int j0 = desc[3 * w + 1], j1 = desc[3 * w + 2];
+	const double sum = NARROW ? wave_row_sum(col, val, x, j0, j1) : wave_row_sum_pairs(col, val, x, j0, j1);
 	if (lane == 0)
 		partials[w] = sum;
 }
@@ -275,9 +278,14 @@ int csr_vec_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 {
 	if (f->m == 0)
 		return SPMVHIP_OK;
+	const bool narrow = f->p.narrow_loads != 0;
 	if (f->nsegs > 0) {
-		csr_split_kernel<<<grid_for(f->nsegs, kWavesPerBlock), kBlock, 0, s>>>(f->col.as<int32_t>(),
-				f->val.as<double>(), x, f->seg_desc.as<int32_t>(), f->nsegs, f->partials.as<double>());
+		if (narrow)
+			csr_split_kernel<true><<<grid_for(f->nsegs, kWavesPerBlock), kBlock, 0, s>>>(f->col.as<int32_t>(),
+					f->val.as<double>(), x, f->seg_desc.as<int32_t>(), f->nsegs, f->partials.as<double>());
+		else
+			csr_split_kernel<false><<<grid_for(f->nsegs, kWavesPerBlock), kBlock, 0, s>>>(f->col.as<int32_t>(),
+					f->val.as<double>(), x, f->seg_desc.as<int32_t>(), f->nsegs, f->partials.as<double>());
 		SPMVHIP_CHECK(hipGetLastError());
 	}
 	const int remap = xcd_chunk(f->p.xcd_remap);
@@ -306,16 +314,26 @@ int csr_vec_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 			LAUNCH_PIPE(1024);
 #undef LAUNCH_PIPE
 	} else {
-#define LAUNCH_BINNED(B)                                                                                    \
-	csr_binned_kernel<B><<<grid, kBlock, 0, s>>>(f->row_ptr.as<int32_t>(), f->col.as<int32_t>(),        \
+#define LAUNCH_BINNED(B, N)                                                                                 \
+	csr_binned_kernel<B, N><<<grid, kBlock, 0, s>>>(f->row_ptr.as<int32_t>(), f->col.as<int32_t>(),     \
 			f->val.as<double>(), x, y, f->blocks.as<int32_t>(), f->nblocks, f->split_nnz, f->lanes, \
 			remap)
-		if (f->block_nnz <= 256)
-			LAUNCH_BINNED(256);
-		else if (f->block_nnz <= 512)
-			LAUNCH_BINNED(512);
-		else
-			LAUNCH_BINNED(1024);
+		if (f->block_nnz <= 256) {
+			if (narrow)
+				LAUNCH_BINNED(256, true);
+			else
+				LAUNCH_BINNED(256, false);
+		} else if (f->block_nnz <= 512) {
+			if (narrow)
+				LAUNCH_BINNED(512, true);
+			else
+				LAUNCH_BINNED(512, false);
+		} else {
+			if (narrow)
+				LAUNCH_BINNED(1024, true);
+			else
+				LAUNCH_BINNED(1024, false);
+		}
 #undef LAUNCH_BINNED
 	}
 	SPMVHIP_CHECK(hipGetLastError());

@@ -58,6 +58,51 @@ __device__ __forceinline__ double lds_strided_sum(const double *p, int e, int b,
 	return (s0 + s1) + (s2 + s3);
 }
 
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+
+// Two adjacent stored entries [j, j + 1] (j even) with one 8-byte column
+// load and one 16-byte value load. The CSR arrays of a converted matrix carry
+// a zeroed tail of kCsrPad entries, so j + 1 may equal nnz.
+
+__device__ __forceinline__ void load_pair(const int32_t *__restrict__ col, const double *__restrict__ val, int j,
+		i32x2 &c, f64x2 &v)
+{
+	c = __builtin_nontemporal_load((const i32x2 *) (col + j));
+	v = __builtin_nontemporal_load((const f64x2 *) (val + j));
+}
+
+// y[r] = sum of row r over [j0, j1), one wave. Lane l takes the entry pairs
+// starting at the even index jb + 2 l, jb + 2 l + 128, ... (jb = j0 rounded
+// down to even), entries outside [j0, j1) masked; two pairs in flight.
+__device__ __forceinline__ double wave_row_sum_pairs(const int32_t *__restrict__ col,
+		const double *__restrict__ val, const double *__restrict__ x, int j0, int j1)
+{
+	const int lane = lane_id();
+	double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+	int j = (j0 & ~1) + 2 * lane;
+	for (; j + 2 * kWave + 1 < j1; j += 4 * kWave) {
+		i32x2 c0, c1;
+		f64x2 v0, v1;
+		load_pair(col, val, j, c0, v0);
+		load_pair(col, val, j + 2 * kWave, c1, v1);
+		const double x0 = x[c0.x], x1 = x[c0.y], x2 = x[c1.x], x3 = x[c1.y];
+		a0 = j >= j0 ? fma(v0.x, x0, a0) : a0;
+		a1 = fma(v0.y, x1, a1);
+		a2 = fma(v1.x, x2, a2);
+		a3 = fma(v1.y, x3, a3);
+	}
+	for (; j < j1; j += 2 * kWave) {
+		i32x2 c;
+		f64x2 v;
+		load_pair(col, val, j, c, v);
+		const double x0 = x[c.x], x1 = x[c.y];
+		a0 = j >= j0 ? fma(v.x, x0, a0) : a0;
+		a1 = j + 1 < j1 ? fma(v.y, x1, a1) : a1;
+	}
+	return wave_sum((a0 + a1) + (a2 + a3));
+}
+
 // y[r] = sum of row r over [j0, j1), one wave, lanes striding the row.
 __device__ __forceinline__ double wave_row_sum(const int32_t *__restrict__ col, const double *__restrict__ val,
 		const double *__restrict__ x, int j0, int j1)
@@ -126,7 +171,15 @@ __device__ __forceinline__ void block_rows_reduce(const double *p, int ra, int n
 //
 // csr_block_at takes the block's nonzero range [j0, j1) = [rp[r0], rp[r1])
 // already known (a precomputed work-unit record); csr_block loads it.
-template <int BNNZ_MAX>
+//
+// Stream layouts: NARROW, one entry per lane and load (entry e0 + 64 u +
+// lane); otherwise two adjacent entries per lane and load, lane l taking the
+// pairs at b + 2 l and b + 2 l + 128 of each 256-entry step from the even
+// index b = j0 rounded down (a 16-byte value load and an 8-byte column load
+// per pair: 8 vector-memory instructions per 256 entries instead of 12, the
+// address unit's work per nonzero cut by a third). Lanes past the block
+// re-read its last pair; products outside [j0, j1) are not stored.
+template <int BNNZ_MAX, bool NARROW = false>
 __device__ __forceinline__ void csr_block_at(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
 		const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y, int r0, int r1,
 		int j0, int j1, int split_nnz, bool skip_split, int lanes_cfg, double *p)
@@ -141,7 +194,7 @@ __device__ __forceinline__ void csr_block_at(const int32_t *__restrict__ rp, con
 	if (nrows == 1 && nz > BNNZ_MAX) {
 		if (skip_split && nz > split_nnz)
 			return;
-		double sum = wave_row_sum(col, val, x, j0, j1);
+		double sum = NARROW ? wave_row_sum(col, val, x, j0, j1) : wave_row_sum_pairs(col, val, x, j0, j1);
 		if (lane == 0)
 			y[r0] = sum;
 		return;
@@ -150,6 +203,37 @@ __device__ __forceinline__ void csr_block_at(const int32_t *__restrict__ rp, con
 	// past the block read its end): one unconditional load, in flight during
 	// the stream phase; row r ends where row r + 1 starts
 	const int ra = rp[r0 + (lane < nrows ? lane : nrows)] - j0;
+	if (!NARROW) {
+		// stream phase, entry pairs: p[j - j0] for j in [j0, j1) (nz < 2 *
+		// BNNZ_MAX for a multi-row block, so p stays inside the buffer)
+		const int jl = (j1 - 1) & ~1; // the block's last pair (used only when j1 > b >= 0)
+		for (int b = j0 & ~1; b < j1; b += 4 * kWave) {
+			i32x2 c[2];
+			f64x2 v[2];
+#pragma unroll
+			for (int u = 0; u < 2; u++) {
+				const int j = b + u * 2 * kWave + 2 * lane;
+				load_pair(col, val, j < jl ? j : jl, c[u], v[u]);
+			}
+			double xv[4];
+#pragma unroll
+			for (int u = 0; u < 2; u++) {
+				xv[2 * u] = x[c[u].x];
+				xv[2 * u + 1] = x[c[u].y];
+			}
+#pragma unroll
+			for (int u = 0; u < 2; u++) {
+				const int j = b + u * 2 * kWave + 2 * lane;
+				if (j >= j0 && j < j1)
+					p[j - j0] = v[u].x * xv[2 * u];
+				if (j + 1 >= j0 && j + 1 < j1)
+					p[j + 1 - j0] = v[u].y * xv[2 * u + 1];
+			}
+		}
+		wave_lds_sync();
+		block_rows_reduce(p, ra, nrows, nz, lanes_cfg, y + r0);
+		return;
+	}
 	// stream phase: products to LDS, coalesced over the block's nnz (nz < 2 *
 	// BNNZ_MAX for a multi-row block, so p[e] stays inside the buffer)
 	for (int e0 = 0; e0 < nz; e0 += 4 * kWave) {
@@ -176,14 +260,15 @@ __device__ __forceinline__ void csr_block_at(const int32_t *__restrict__ rp, con
 	block_rows_reduce(p, ra, nrows, nz, lanes_cfg, y + r0);
 }
 
-template <int BNNZ_MAX>
+template <int BNNZ_MAX, bool NARROW = false>
 __device__ __forceinline__ void csr_block(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
 		const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y, int r0, int r1,
 		int split_nnz, bool skip_split, int lanes_cfg, double *p)
 {
 	r0 = __builtin_amdgcn_readfirstlane(r0);
 	r1 = __builtin_amdgcn_readfirstlane(r1);
-	csr_block_at<BNNZ_MAX>(rp, col, val, x, y, r0, r1, rp[r0], rp[r1], split_nnz, skip_split, lanes_cfg, p);
+	csr_block_at<BNNZ_MAX, NARROW>(rp, col, val, x, y, r0, r1, rp[r0], rp[r1], split_nnz, skip_split, lanes_cfg,
+			p);
 }
 
 // The same row block split in three stages, so that a persistent wave can

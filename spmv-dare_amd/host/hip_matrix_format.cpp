@@ -13,7 +13,7 @@
 //                     stay on the GPU; y is copied back by statistics_print_data()
 //                     and when the object is destroyed
 //   SELLCS_C, SELLCS_SIGMA          SELL-C-sigma slice height / sort window
-//   CSRV_BLOCK_NNZ, CSRV_LANES, CSRV_SPLIT_NNZ   CSR-vector row-block tuning
+//   CSRV_BLOCK_NNZ, CSRV_LANES, CSRV_SPLIT_NNZ, CSRV_NARROW_LOADS   CSR-vector row-block tuning
 //   SPMV_XCD_REMAP, SELL_UNROLL, SELL_PIPELINE, SELL_X_WINDOW, SELL_Y_MODE
 //   DIV_THREADS, DIV_LANES, CSRCV_NUM_PACKET_VALS   DIV-RF codec parameters
 
@@ -115,6 +115,7 @@ struct Matrix_Format *csr_to_format(INT_T *row_ptr, INT_T *col_ind, ValueTypeRef
 	p.div_threads = (int) env_long("DIV_THREADS", 0);
 	p.div_lanes = (int) env_long("DIV_LANES", 0);
 	p.div_acc_copies = (int) env_long("DIV_ACC_COPIES", 0);
+	p.narrow_loads = (int) env_long("CSRV_NARROW_LOADS", 0);
 	p.div_packet_vals = env_long("CSRCV_NUM_PACKET_VALS", 0);
 
 	HipMatrixFormat *f = new HipMatrixFormat(m, n, nnz);

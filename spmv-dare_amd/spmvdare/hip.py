@@ -30,7 +30,8 @@ class Params(ctypes.Structure):
                 ("x_window", ctypes.c_int), ("unroll", ctypes.c_int),
                 ("div_threads", ctypes.c_int), ("div_lanes", ctypes.c_int),
                 ("div_packet_vals", ctypes.c_long), ("pipeline", ctypes.c_int),
-                ("y_mode", ctypes.c_int), ("div_acc_copies", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
+                ("y_mode", ctypes.c_int), ("div_acc_copies", ctypes.c_int), ("narrow_loads", ctypes.c_int),
+                ("reserved", ctypes.c_int * 4)]
 
 
 class Info(ctypes.Structure):

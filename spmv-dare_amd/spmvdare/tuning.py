@@ -53,6 +53,15 @@ def _variants():
 
 VARIANTS = _variants()
 assert len(VARIANTS) == 40
+# the plain per-matrix CSR launch the batched path is compared with
+DEFAULT_CSR = VARIANTS.index(("csr_vec", {"block_nnz": 256, "lanes_per_row": 0}))
+
+
+def batch_allowed():
+    """Variants a member of the persistent batched launch may use: the launch
+    stages CSR row blocks of at most 256 nonzeros at full occupancy (a
+    512-nonzero member would halve it for every member)."""
+    return np.array([not (k == "csr_vec" and kw.get("block_nnz", 256) > 256) for k, kw in VARIANTS])
 
 
 def variant_name(i):
