@@ -95,7 +95,15 @@ typedef struct spmvhip_params {
 	                         value loads); 0 (default) = two adjacent entries per lane and load
 	                         (8-byte column, 16-byte value loads: a third fewer vector-memory
 	                         instructions per nonzero) */
-	int reserved[4];
+	int div_layout;       /* DIV_RF: device packet layout. 1 = the reference's packet bytes; 2 = x
+	                         slots (each packet's distinct columns, ascending, are numbered,
+	                         coordinates hold the slot, and the decoder stages x for every slot into
+	                         LDS with loads along runs of consecutive columns); 3 = interleaved lane
+	                         streams (the L value streams as rows of L dwords, so the lanes' stream
+	                         reads fall in distinct LDS banks); 4 = both. 0 (default) = automatic:
+	                         the fastest measured layout whose packet fits the LDS, else 1.
+	                         Explicit 2..4 are an error when a packet does not fit */
+	int reserved[3];
 } spmvhip_params;
 
 typedef struct spmvhip_format spmvhip_format;   /* opaque */
@@ -265,6 +273,17 @@ void spmvhip_mlp_destroy(spmvhip_mlp *net);
 int spmvhip_div_encode(const int32_t *row_ptr, const int32_t *col_ind, const double *values,
 		long m, long n, long nnz, int T, int L, long P, uint8_t *bytes, size_t *nbytes,
 		int64_t *pk_off, long *npackets, int32_t *grp_pk, long *ngroups);
+
+/* The device form of the same stream in device layout `layout` (the
+ * div_layout codes 1..4 of spmvhip_params: 1 the reference bytes, 2 x slots,
+ * 3 interleaved lane streams, 4 both), as spmv-dare_amd/csrc/div/div_codec.hpp
+ * describes, each packet at a 16-byte aligned offset. Two calls as
+ * spmvhip_div_encode: bytes[*nbytes], pk_off[*npackets + 1]. No reference
+ * counterpart (the reference's DIV decoder reads its own packets); needs no
+ * device. */
+int spmvhip_div_device_encode(const int32_t *row_ptr, const int32_t *col_ind, const double *values,
+		long m, long n, long nnz, int T, int L, long P, int layout, uint8_t *bytes, size_t *nbytes,
+		int64_t *pk_off, long *npackets);
 
 #ifdef __cplusplus
 }

@@ -377,11 +377,10 @@ uint64_t get_le(const uint8_t *p, int nbytes)
 }
 
 struct SlotSizes {
-	size_t runs_at, bits_at, coords_at, lens_at, streams_at, total;
+	size_t runs_at, bits_at, coords_at, lens_at, streams_at;
 };
 
-SlotSizes slot_sizes(int L, uint32_t nv, uint32_t nu, uint32_t nrfs, uint32_t nruns, uint32_t nslots, int cb,
-		size_t stream_bytes)
+SlotSizes slot_sizes(int L, uint32_t nv, uint32_t nu, uint32_t nrfs, uint32_t nruns, uint32_t nslots, int cb)
 {
 	SlotSizes z;
 	z.runs_at = (size_t) slot_header_bytes(L) + 8 * (size_t) nrfs;
@@ -389,13 +388,41 @@ SlotSizes slot_sizes(int L, uint32_t nv, uint32_t nu, uint32_t nrfs, uint32_t nr
 	z.coords_at = z.bits_at + 8 * (((size_t) nslots + 63) / 64);
 	z.lens_at = z.coords_at + (size_t) nv * cb;
 	z.streams_at = z.lens_at + nu;
-	z.total = z.streams_at + stream_bytes;
 	return z;
+}
+
+// The lane streams of a packet (lane sizes from its header), contiguous or
+// interleaved by dwords, appended to `o` from byte `at`; returns the end.
+size_t put_streams(std::vector<uint8_t> &o, size_t at, const uint8_t *lane_sizes, const uint8_t *streams, int L,
+		bool interleave)
+{
+	size_t total = 0, maxw = 0;
+	for (int l = 0; l < L; l++) {
+		const size_t sz = get32(lane_sizes + 4 * l);
+		total += sz;
+		maxw = std::max(maxw, (sz + 3) / 4);
+	}
+	if (interleave)
+		at = (at + 3) & ~(size_t) 3; // dword-aligned rows
+	const size_t end = at + (interleave ? maxw * 4 * (size_t) L : total);
+	o.resize((end + 15) / 16 * 16, 0);
+	if (!interleave) {
+		memcpy(&o[at], streams, total);
+		return end;
+	}
+	size_t base = 0;
+	for (int l = 0; l < L; l++) {
+		const size_t sz = get32(lane_sizes + 4 * l);
+		for (size_t b = 0; b < sz; b++)
+			o[at + (b >> 2) * 4 * (size_t) L + 4 * (size_t) l + (b & 3)] = streams[base + b];
+		base += sz;
+	}
+	return end;
 }
 
 } // namespace
 
-int to_slot_layout(const Stream &st, SlotLayout *out, const char **err)
+int to_device_layout(const Stream &st, bool slots, bool interleave, DeviceLayout *out, const char **err)
 {
 	const long np = st.npackets();
 	const int L = st.L;
@@ -412,6 +439,17 @@ int to_slot_layout(const Stream &st, SlotLayout *out, const char **err)
 		const int cb = (row_bits + col_bits + 7) >> 3;
 		const size_t hb = (size_t) header_bytes(L);
 		const uint8_t *coords = src + hb + 8 * (size_t) nrfs;
+		const uint8_t *lens = coords + (size_t) nv * cb;
+		std::vector<uint8_t> &o = pk[p];
+		max_rows = std::max(max_rows, nrows);
+		if (!slots) {
+			const size_t streams_at = (size_t) (lens - src) + nu;
+			o.assign(streams_at, 0);
+			memcpy(o.data(), src, streams_at);
+			o[22] = interleave ? 1 : 0;
+			put_streams(o, streams_at, src + kHeaderFixed, src + streams_at, L, interleave);
+			continue;
+		}
 		const uint64_t rmask = row_bits ? (1ull << row_bits) - 1 : 0;
 		std::vector<uint32_t> rows(nv), cols(nv);
 		for (uint32_t k = 0; k < nv; k++) {
@@ -442,12 +480,8 @@ int to_slot_layout(const Stream &st, SlotLayout *out, const char **err)
 			continue;
 		}
 		const int ncb = std::max(1, (rb + sb + 7) >> 3);
-		size_t stream_bytes = 0;
-		for (int l = 0; l < L; l++)
-			stream_bytes += get32(src + kHeaderFixed + 4 * l);
-		const SlotSizes z = slot_sizes(L, nv, nu, nrfs, (uint32_t) run_col.size(), nslots, ncb, stream_bytes);
-		std::vector<uint8_t> &o = pk[p];
-		o.assign((z.total + 15) / 16 * 16, 0);
+		const SlotSizes z = slot_sizes(L, nv, nu, nrfs, (uint32_t) run_col.size(), nslots, ncb);
+		o.assign(z.streams_at, 0);
 		put32(o, 0, nv);
 		put32(o, 4, nu);
 		put32(o, 8, nrows);
@@ -456,6 +490,7 @@ int to_slot_layout(const Stream &st, SlotLayout *out, const char **err)
 		o[20] = (uint8_t) rb;
 		o[21] = (uint8_t) sb;
 		o[22] = (uint8_t) ncb;
+		o[23] = interleave ? 1 : 0;
 		put32(o, 24, nrfs);
 		put32(o, 28, (uint32_t) run_col.size());
 		memcpy(&o[kSlotHeaderFixed], src + kHeaderFixed, 4 * (size_t) L);            // lane sizes
@@ -470,10 +505,9 @@ int to_slot_layout(const Stream &st, SlotLayout *out, const char **err)
 			for (int b = 0; b < ncb; b++)
 				o[z.coords_at + (size_t) k * ncb + b] = (uint8_t) (c >> (8 * b));
 		}
-		const uint8_t *lens = coords + (size_t) nv * cb;
-		memcpy(&o[z.lens_at], lens, nu + stream_bytes); // val_lens and lane streams, unchanged
+		memcpy(&o[z.lens_at], lens, nu); // val_lens, unchanged
+		put_streams(o, z.streams_at, src + kHeaderFixed, lens + nu, L, interleave);
 		max_slots = std::max(max_slots, nslots);
-		max_rows = std::max(max_rows, nrows);
 	}
 	if (fail)
 		return -1;
@@ -494,32 +528,6 @@ int to_slot_layout(const Stream &st, SlotLayout *out, const char **err)
 	out->max_slots = max_slots;
 	out->max_rows = max_rows;
 	return 0;
-}
-
-void slot_packet_coords(const uint8_t *pk, int L, int32_t *rows, int32_t *cols)
-{
-	const uint32_t nv = get32(pk), nu = get32(pk + 4), row_min = get32(pk + 12), nslots = get32(pk + 16);
-	const int rb = pk[20], cb = pk[22];
-	const uint32_t nrfs = get32(pk + 24), nruns = get32(pk + 28);
-	const SlotSizes z = slot_sizes(L, nv, nu, nrfs, nruns, nslots, cb, 0);
-	std::vector<uint32_t> slot_col(nslots);
-	long r = -1;
-	uint32_t start = 0;
-	for (uint32_t s = 0; s < nslots; s++) {
-		uint64_t w;
-		memcpy(&w, pk + z.bits_at + 8 * (s >> 6), 8);
-		if (w >> (s & 63) & 1) {
-			r++;
-			start = s;
-		}
-		slot_col[s] = get32(pk + z.runs_at + 4 * r) + (s - start);
-	}
-	const uint64_t rmask = rb ? (1ull << rb) - 1 : 0;
-	for (uint32_t k = 0; k < nv; k++) {
-		const uint64_t c = get_le(pk + z.coords_at + (size_t) k * cb, cb);
-		rows[k] = (int32_t) (row_min + (uint32_t) (c & rmask));
-		cols[k] = (int32_t) slot_col[c >> rb];
-	}
 }
 
 } // namespace spmvdiv

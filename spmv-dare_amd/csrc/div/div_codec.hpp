@@ -42,40 +42,47 @@ struct Stream {
 int encode(const int32_t *row_ptr, const int32_t *col, const double *val, long m, long n, long nnz,
 		int T, int L, long P, Stream *out, const char **err);
 
-// x-slot device layout of a packet (the GPU decoder's form of a reference
-// packet; values, lengths and lane streams are the reference's bytes).
-// The packet's distinct columns, in ascending order, are its x slots; they
-// form runs of consecutive columns. A coordinate is row_diff | slot <<
-// row_bits, and the decoder stages x[column of slot s] for every slot into
-// LDS with loads that follow the runs, so its x reads are contiguous instead
-// of one cache line per value. Layout (little endian, sections in order):
+// Device layouts of a packet (the GPU decoders' forms of a reference packet;
+// values and value lengths are the reference's bytes in every form).
+//
+// x slots: the packet's distinct columns, in ascending order, are its x
+// slots; they form runs of consecutive columns. A coordinate is row_diff |
+// slot << row_bits, and the decoder stages x[column of slot s] for every slot
+// into LDS with loads that follow the runs, so its x reads are contiguous
+// instead of one cache line per value. Layout (little endian, in order):
 //   u32 num_vals, num_vals_unique, num_rows, row_min, num_slots
-//   u8 row_bits, slot_bits, coord_bytes, 0; u32 num_rfs, num_runs
+//   u8 row_bits, slot_bits, coord_bytes, interleaved; u32 num_rfs, num_runs
 //   u32 lane_size[L]                      (the reference header's)
 //   u32 rfs[num_rfs]; u32 unique_per_rf[num_rfs]
 //   u32 run_col[num_runs]                 first column of each run
 //   (8-byte aligned) u64 run_start_bits[ceil(num_slots / 64)]: bit s set
 //                                         where slot s starts a run
 //   (4-byte aligned) coords: num_vals x coord_bytes, in the reference's order
-//   u8 val_lens[num_vals_unique]; L lane byte streams (the reference's)
+//   u8 val_lens[num_vals_unique]; the lane value streams
+// Without x slots the packet is the reference's, header byte 22 (padding in
+// the reference) holding the interleaved flag.
+//
+// Interleaved lane streams: the L value streams as rows of L dwords, lane l's
+// stream dword w at byte 4 (w L + l) of the stream section, which starts at
+// the next 4-byte aligned offset after the value lengths (rows up to the
+// longest stream, zero filled), so that the lanes' reads of their streams
+// fall in distinct LDS banks, and a lane's stream base needs no scan of the
+// lane sizes. Otherwise the streams follow each other, lane 0 first.
 constexpr int kSlotHeaderFixed = 32;
 
 inline int slot_header_bytes(int L) { return kSlotHeaderFixed + 4 * L; }
 
-struct SlotLayout {
+struct DeviceLayout {
 	std::vector<uint8_t> bytes;   // packets, each starting at a 16-byte aligned offset
 	std::vector<int64_t> pk_off;  // device byte offset of each packet, + total
 	int64_t max_packet = 0;       // largest packet, 16-byte rounded
 	uint32_t max_slots = 0, max_rows = 0;
 };
 
-// Transcode every packet of `st` (lanes L) into the x-slot layout. Returns
-// 0, or -1 with *err set when a packet's coordinate exceeds 32 bits.
-int to_slot_layout(const Stream &st, SlotLayout *out, const char **err);
-
-// The (row, column) of every value of one x-slot packet, in packet order
-// (what the reference's coordinate section holds): rows[k], cols[k] for
-// k < num_vals. For checks of the transcoding.
-void slot_packet_coords(const uint8_t *pk, int L, int32_t *rows, int32_t *cols);
+// Transcode every packet of `st` (lanes L) into a device layout: x slots
+// and/or interleaved lane streams (both false: the reference bytes, each
+// packet 16-byte aligned). Returns 0, or -1 with *err set when an x-slot
+// coordinate exceeds 32 bits.
+int to_device_layout(const Stream &st, bool slots, bool interleave, DeviceLayout *out, const char **err);
 
 } // namespace spmvdiv

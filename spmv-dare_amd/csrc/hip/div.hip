@@ -50,6 +50,20 @@ __device__ __forceinline__ uint64_t load_le(const uint8_t *p, int nbytes)
 	return v;
 }
 
+// Byte b of lane `lane`'s value stream (the byte-at-a-time decode of the
+// export kernels): interleaved (row bytes lrow) or contiguous from sbase.
+__device__ __forceinline__ uint64_t stream_le(const uint8_t *pk, long sbase, int lane, long b, int nbytes, bool il,
+		int lrow)
+{
+	uint64_t v = 0;
+	for (int k = 0; k < nbytes; k++) {
+		const long q = b + k;
+		const uint8_t byte = il ? pk[sbase + (q >> 2) * lrow + 4 * lane + (q & 3)] : pk[sbase + q];
+		v |= (uint64_t) byte << (8 * k);
+	}
+	return v;
+}
+
 __device__ __forceinline__ void wave_fence()
 {
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -61,7 +75,7 @@ __device__ __forceinline__ void wave_fence()
 // parity checks of the device decoder.
 __global__ void __launch_bounds__(kBlock) div_export_kernel(const uint8_t *__restrict__ bytes,
 		const int64_t *__restrict__ pk_off, const int64_t *__restrict__ pk_nnz, const int32_t *__restrict__ grp_pk,
-		long ngroups, int L, int32_t *__restrict__ rows_out, int32_t *__restrict__ cols_out,
+		long ngroups, int L, int il, int32_t *__restrict__ rows_out, int32_t *__restrict__ cols_out,
 		double *__restrict__ vals_out)
 {
 	const int lane = lane_id(), wv = threadIdx.x / kWave;
@@ -89,7 +103,7 @@ __global__ void __launch_bounds__(kBlock) div_export_kernel(const uint8_t *__res
 		const uint32_t *per_rf = rfs + nrfs;
 		const uint8_t *coords = pk + 28 + 4 * L + 8 * (long) nrfs;
 		const uint8_t *lens = coords + (long) nv * cb;
-		const uint8_t *stream = lens + nu + lane_base;
+		const long sbase = il ? ((lens - pk) + nu + 3) & ~3L : (lens - pk) + nu + lane_base;
 		const long nnz0 = pk_nnz[p];
 		long cursor = 0, ks = 0, cs = 0;
 		for (uint32_t gi = 0; gi < nrfs; gi++) {
@@ -102,7 +116,7 @@ __global__ void __launch_bounds__(kBlock) div_export_kernel(const uint8_t *__res
 					const uint8_t lb = lens[ks + pp * L + lane];
 					const int len = lb & 15, tz = (lb >> 4) << 2;
 					if (len) {
-						val += load_le(stream + cursor, len) << tz;
+						val += stream_le(pk, sbase, lane, cursor, len, il, 4 * L) << tz;
 						cursor += len;
 					}
 					double d;
@@ -295,6 +309,29 @@ __device__ __forceinline__ uint32_t lds_le_b32x(const uint8_t *base, int pos, ui
 	return __builtin_amdgcn_alignbyte(w[1], w[0], (uint32_t) (pos & 3)) & mask;
 }
 
+// A lane's next delta: nbytes (0..8) at byte `cursor` of its value stream,
+// which starts at sbase. IL: the streams are interleaved by dwords (lane l's
+// dword w at sbase + w * lrow, sbase = stream section + 4 l, lrow = 4 L;
+// LROW is lrow when known at compile time), so the lanes' three dword reads
+// fall in distinct banks; the reads past a stream's end stay inside the
+// packet buffer's two rows of slack. Otherwise the stream is contiguous.
+template <bool IL, int LROW>
+__device__ __forceinline__ uint64_t stream_read(const uint8_t *pk, int sbase, int cursor, int nbytes, int lrow)
+{
+	if (!IL)
+		return lds_le_b32(pk, sbase + cursor, nbytes);
+	const int row = LROW ? LROW : lrow;
+	const uint32_t *w = (const uint32_t *) (pk + sbase + (cursor >> 2) * row);
+	const uint32_t sh = (uint32_t) (cursor & 3);
+	const uint32_t w0 = w[0], w1 = w[row / 4], w2 = w[row / 2];
+	uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, sh);
+	uint32_t hi = __builtin_amdgcn_alignbyte(w2, w1, sh);
+	lo &= nbytes >= 4 ? 0xffffffffu : (1u << (8 * nbytes)) - 1;
+	hi &= nbytes >= 8 ? 0xffffffffu : nbytes > 4 ? (1u << (8 * (nbytes - 4))) - 1 : 0u;
+	return ((uint64_t) hi << 32) | lo;
+}
+
+
 // One replication-factor-1 group of a packet staged in LDS (the common case:
 // every value distinct). C4: the coordinates fit 4 bytes and are read with
 // one byte-align.
@@ -332,7 +369,7 @@ __device__ __forceinline__ uint32_t div_coord(const uint8_t *pk, int at, int cb,
 // scheduling barriers keep them together): the compiler does not move loads
 // across the LDS atomics, so reads between the atomics would put one LDS
 // round trip per value on the critical path.
-template <int K, int S, bool C4, bool A4L64>
+template <int K, int S, bool C4, bool A4L64, bool IL>
 __device__ __forceinline__ void div_full_steps(const uint8_t *__restrict__ pk, int &at, int &la, int cstride,
 		int lstride, int cb, uint32_t cmask, int row_bits, uint32_t col_min, int stream_at,
 		const double *__restrict__ x, double *__restrict__ yw, int &cursor, uint64_t &val)
@@ -363,7 +400,7 @@ __device__ __forceinline__ void div_full_steps(const uint8_t *__restrict__ pk, i
 #pragma unroll
 		for (int k = 0; k < K; k++) {
 			const int len = lb[k] & 15, tz = (lb[k] >> 4) << 2;
-			dlt[k] = lds_le_b32(pk, stream_at + cursor, len) << tz;
+			dlt[k] = stream_read<IL, A4L64 ? 4 * kWave : 0>(pk, stream_at, cursor, len, 4 * lstride) << tz;
 			cursor += len;
 		}
 		__builtin_amdgcn_sched_barrier(0); // then all deltas, before the first accumulation
@@ -383,7 +420,7 @@ __device__ __forceinline__ void div_full_steps(const uint8_t *__restrict__ pk, i
 // 4-byte alignment, every coordinate is dword-aligned) and it has 64 lanes.
 // The full steps then read each coordinate with one aligned LDS dword read
 // and each length byte at a compile-time stride, with immediate offsets.
-template <int K, bool C4, bool A4L64 = false>
+template <int K, bool C4, bool A4L64, bool IL>
 __device__ __forceinline__ void div_rf1_group(const uint8_t *__restrict__ pk, int coords_at, int lens_at, int lens0,
 		int crd0_at, int stream_at, int steps, int cnt, int dv, int L, int cb, uint32_t col_min, int row_bits,
 		const double *__restrict__ x, double *__restrict__ yw, int &cursor)
@@ -403,14 +440,14 @@ __device__ __forceinline__ void div_rf1_group(const uint8_t *__restrict__ pk, in
 			// K = 8): both steps' gathers are requested before either is
 			// decoded, so the second step's gathers wait behind no decode
 			if (nfull == 2 * K) {
-				div_full_steps<K, 2, C4, A4L64>(pk, at, la, cstride, lstride, cb, cmask, row_bits, col_min,
+				div_full_steps<K, 2, C4, A4L64, IL>(pk, at, la, cstride, lstride, cb, cmask, row_bits, col_min,
 						stream_at, x, yw, cursor, val);
 				p0 = nfull;
 			}
 		}
 		for (; p0 < nfull; p0 += K)
-			div_full_steps<K, 1, C4, A4L64>(pk, at, la, cstride, lstride, cb, cmask, row_bits, col_min, stream_at,
-					x, yw, cursor, val);
+			div_full_steps<K, 1, C4, A4L64, IL>(pk, at, la, cstride, lstride, cb, cmask, row_bits, col_min,
+					stream_at, x, yw, cursor, val);
 	}
 	// the rest (at most K + 1 values per lane), each value tested
 	for (int p0 = nfull; p0 < steps; p0 += K) {
@@ -430,7 +467,7 @@ __device__ __forceinline__ void div_rf1_group(const uint8_t *__restrict__ pk, in
 			(void) div_coord<K, C4>(pk, ok ? crd0_at + (p0 + k) * cstride : coords_at, cb, cmask, row_bits, row);
 			const int lb = pk[ok ? lens0 + (p0 + k) * L : lens_at];
 			const int len = ok ? lb & 15 : 0, tz = (lb >> 4) << 2;
-			val += lds_le_b32(pk, stream_at + cursor, len) << tz;
+			val += stream_read<IL, 0>(pk, stream_at, cursor, len, 4 * L) << tz;
 			cursor += len;
 			double d;
 			__builtin_memcpy(&d, &val, 8);
@@ -440,6 +477,8 @@ __device__ __forceinline__ void div_rf1_group(const uint8_t *__restrict__ pk, in
 	}
 }
 
+// The automatic device layout's order of preference (div_layout codes).
+constexpr int kDivAutoLayouts[] = {3, 1};
 constexpr int kDivAccCopiesMax = 64;    // copies of the LDS row accumulators (a power of two)
 // The automatic choice: up to 4 copies within 4 KiB. Measured on the 8M-row
 // matrix (packets of at most 48 rows): 1 copy 0.89 ms, 4 copies 0.747 ms,
@@ -457,7 +496,7 @@ __device__ __forceinline__ double div_acc_sum(const double *yw, int stride, int 
 	return s;
 }
 
-template <int K>
+template <int K, bool IL>
 __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__restrict__ bytes,
 		const int64_t *__restrict__ pk_off, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
 		long ngroups, int L, int buf_bytes, int acc_stride, int acc_copies, int xcd_k, const double *__restrict__ x,
@@ -507,18 +546,24 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 		const uint32_t nrfs = h[6];
 		const int cb = (row_bits + col_bits + 7) >> 3;
 		const uint64_t row_mask = row_bits ? ((1ull << row_bits) - 1) : 0;
-		long my_size = lane < L ? (long) h[7 + lane] : 0;
-		long incl = my_size;
-		for (int off = 1; off < kWave; off <<= 1) {
-			long o = __shfl_up(incl, off, kWave);
-			if (lane >= off)
-				incl += o;
-		}
 		const uint32_t *rfs = h + 7 + L;
 		const uint32_t *per_rf = rfs + nrfs;
 		const long coords_at = 28 + 4 * (long) L + 8 * (long) nrfs;
 		const long lens_at = coords_at + (long) nv * cb;
-		const long stream_at = lens_at + nu + (incl - my_size);
+		long stream_at;
+		if (IL) {
+			stream_at = ((lens_at + nu + 3) & ~3L) + 4 * lane; // the lane's column of the interleaved streams
+		} else {
+			// the lane's stream start: exclusive scan of the lane sizes
+			long my_size = lane < L ? (long) h[7 + lane] : 0;
+			long incl = my_size;
+			for (int off = 1; off < kWave; off <<= 1) {
+				long o = __shfl_up(incl, off, kWave);
+				if (lane >= off)
+					incl += o;
+			}
+			stream_at = lens_at + nu + (incl - my_size);
+		}
 		for (int r = lane; r < acc_copies * acc_stride; r += kWave)
 			yw[r] = 0.0;
 		wave_fence();
@@ -533,15 +578,17 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 				int cur = (int) cursor;
 				const int lens0 = (int) (lens_at + ks) + lane, crd0 = (int) cs + lane;
 				if (cb == 4 && L == kWave)
-					div_rf1_group<K, true, true>(pk, (int) coords_at, (int) lens_at, lens0,
+					div_rf1_group<K, true, true, IL>(pk, (int) coords_at, (int) lens_at, lens0,
 							(int) coords_at + crd0 * cb, (int) stream_at, (int) steps, (int) cnt, (int) dv, L, cb,
 							col_min, row_bits, x, ywl, cur);
 				else if (cb <= 4)
-					div_rf1_group<K, true>(pk, (int) coords_at, (int) lens_at, lens0, (int) coords_at + crd0 * cb,
-							(int) stream_at, (int) steps, (int) cnt, (int) dv, L, cb, col_min, row_bits, x, ywl, cur);
+					div_rf1_group<K, true, false, IL>(pk, (int) coords_at, (int) lens_at, lens0,
+							(int) coords_at + crd0 * cb, (int) stream_at, (int) steps, (int) cnt, (int) dv, L, cb,
+							col_min, row_bits, x, ywl, cur);
 				else
-					div_rf1_group<K, false>(pk, (int) coords_at, (int) lens_at, lens0, (int) coords_at + crd0 * cb,
-							(int) stream_at, (int) steps, (int) cnt, (int) dv, L, cb, col_min, row_bits, x, ywl, cur);
+					div_rf1_group<K, false, false, IL>(pk, (int) coords_at, (int) lens_at, lens0,
+							(int) coords_at + crd0 * cb, (int) stream_at, (int) steps, (int) cnt, (int) dv, L, cb,
+							col_min, row_bits, x, ywl, cur);
 				cursor = cur;
 				ks += n;
 				cs += n;
@@ -556,7 +603,7 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 					const uint8_t lb = pk[lens_at + ks + pp * L + lane];
 					const int len = lb & 15, tz = (lb >> 4) << 2;
 					if (len) {
-						val += lds_le(pk, stream_at + cursor, len) << tz;
+						val += stream_read<IL, 0>(pk, (int) stream_at, (int) cursor, len, 4 * L) << tz;
 						cursor += len;
 					}
 					double d;
@@ -586,6 +633,364 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 				y[row_min + r] += div_acc_sum(yw, acc_stride, acc_copies, (int) r);
 		}
 		wave_fence(); // the packet buffer and accumulators are reused
+	}
+}
+
+// ------------------------------------------------------------ x-slot layout
+//
+// The decoder of the x-slot device layout (div_codec.hpp: to_slot_layout).
+// Per packet: (1) the packet is copied to LDS as above; (2) x is staged:
+// slot s of the packet (its s-th smallest distinct column) gets xs[s] =
+// x[column], lane l of a 64-slot chunk taking slot 64 c + l, so the loads of
+// a chunk follow the packet's runs of consecutive columns (a few cache lines
+// per load instead of one per lane); (3) the values are decoded as in
+// div_spmv_lds_kernel, each coordinate giving a row and a slot, x read from
+// xs in LDS.
+//
+// The run of slot s: the bit set of run starts (one u64 word per chunk) is
+// counted up to s. Runs started before chunk c and the start of the run open
+// at the chunk's first slot are carried from chunk to chunk in scalar
+// registers.
+
+struct SlotHeader {
+	uint32_t nv, nu, nrows, row_min, nslots, nrfs, nruns;
+	int rb, cb;
+	int lanes_at, rfs_at, runs_at, bits_at, coords_at, lens_at, streams_at;
+};
+
+// The header of a staged x-slot packet (wave-uniform values, scalar registers).
+__device__ __forceinline__ SlotHeader slot_header(const uint8_t *pk, int L)
+{
+	const uint32_t *h = (const uint32_t *) pk;
+	SlotHeader z;
+	z.nv = __builtin_amdgcn_readfirstlane(h[0]);
+	z.nu = __builtin_amdgcn_readfirstlane(h[1]);
+	z.nrows = __builtin_amdgcn_readfirstlane(h[2]);
+	z.row_min = __builtin_amdgcn_readfirstlane(h[3]);
+	z.nslots = __builtin_amdgcn_readfirstlane(h[4]);
+	const uint32_t b = __builtin_amdgcn_readfirstlane(h[5]);
+	z.rb = (int) (b & 0xff);
+	z.cb = (int) ((b >> 16) & 0xff);
+	z.nrfs = __builtin_amdgcn_readfirstlane(h[6]);
+	z.nruns = __builtin_amdgcn_readfirstlane(h[7]);
+	z.lanes_at = spmvdiv::kSlotHeaderFixed;
+	z.rfs_at = spmvdiv::kSlotHeaderFixed + 4 * L;
+	z.runs_at = z.rfs_at + 8 * (int) z.nrfs;
+	z.bits_at = (z.runs_at + 4 * (int) z.nruns + 7) & ~7;
+	z.coords_at = z.bits_at + 8 * (((int) z.nslots + 63) >> 6);
+	z.lens_at = z.coords_at + (int) z.nv * z.cb;
+	z.streams_at = z.lens_at + (int) z.nu;
+	return z;
+}
+
+// xs[s] = x[column of slot s] for every slot of the staged packet. Eight
+// chunks of 64 slots per round: their run-start words and run columns are
+// read from LDS, then all eight loads of x are requested before the LDS
+// stores. Slots past the last are given the last slot's column (a valid
+// column) and not stored.
+__device__ __forceinline__ void slot_stage_x(const uint8_t *pk, const SlotHeader &z, const double *__restrict__ x,
+		double *xs)
+{
+	const int lane = lane_id();
+	const int nsl = (int) z.nslots;
+	const uint32_t *run_col = (const uint32_t *) (pk + z.runs_at);
+	const uint32_t *bits = (const uint32_t *) (pk + z.bits_at);
+	int run_base = 0, carry = 0; // runs started before the chunk; first slot of the run open at its start
+	for (int c0 = 0; c0 * kWave < nsl; c0 += 8) {
+		double xv[8];
+#pragma unroll
+		for (int i = 0; i < 8; i++) {
+			const int c = c0 + i;
+			xv[i] = 0.0;
+			if (c * kWave < nsl) { // wave-uniform
+				const uint32_t wlo = __builtin_amdgcn_readfirstlane(bits[2 * c]);
+				const uint32_t whi = __builtin_amdgcn_readfirstlane(bits[2 * c + 1]);
+				const uint64_t w = ((uint64_t) whi << 32) | wlo;
+				const int s = min(c * kWave + lane, nsl - 1);
+				const int l = s - c * kWave;
+				const uint64_t mk = w & ((2ull << l) - 1); // run starts at slots <= s of this chunk
+				const int k = __popcll(mk);
+				const int r = run_base + k - 1;
+				const int start = k ? c * kWave + 63 - __clzll(mk) : carry;
+				xv[i] = x[run_col[r] + (uint32_t) (s - start)];
+				run_base += __popcll(w);
+				if (w)
+					carry = c * kWave + 63 - __clzll(w);
+			}
+		}
+#pragma unroll
+		for (int i = 0; i < 8; i++) {
+			const int s = (c0 + i) * kWave + lane;
+			if (s < nsl)
+				xs[s] = xv[i];
+		}
+	}
+}
+
+// One replication-factor-1 group of a staged x-slot packet, K values per
+// lane and step: the step's K coordinates and length bytes, then its K
+// staged x values and deltas, then its K accumulations. Steps where every
+// lane below L has values need no validity tests; later values are tested
+// (their reads are redirected to positions inside the packet).
+template <int K, bool CB2, bool IL>
+__device__ __forceinline__ void slot_rf1_group(const uint8_t *__restrict__ pk, const double *xs, int coords_at,
+		int lens_at, int crd, int la, int cstride, int lstride, int stream_at, int steps, int cnt, int dv, int L,
+		int cb, int rb, double *yw, int &cursor)
+{
+	const uint32_t cmask = cb >= 4 ? 0xffffffffu : (1u << (8 * cb)) - 1;
+	const uint32_t rmask = (1u << rb) - 1;
+	uint64_t val = 0;
+	const int nfull = dv / K * K;
+	int p0 = 0;
+	if (lane_id() < L) {
+		for (; p0 < nfull; p0 += K) {
+			uint32_t c[K];
+			int lb[K];
+#pragma unroll
+			for (int k = 0; k < K; k++) {
+				c[k] = CB2 ? (uint32_t) * (const uint16_t *) (pk + crd + k * cstride)
+				           : lds_le_b32x(pk, crd + k * cstride, cmask);
+				lb[k] = pk[la + k * lstride];
+			}
+			crd += K * cstride;
+			la += K * lstride;
+			__builtin_amdgcn_sched_barrier(0);
+			double xv[K];
+			uint64_t dlt[K];
+#pragma unroll
+			for (int k = 0; k < K; k++)
+				xv[k] = xs[c[k] >> rb];
+#pragma unroll
+			for (int k = 0; k < K; k++) {
+				const int len = lb[k] & 15, tz = (lb[k] >> 4) << 2;
+				dlt[k] = stream_read<IL, 0>(pk, stream_at, cursor, len, 4 * L) << tz;
+				cursor += len;
+			}
+			__builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+			for (int k = 0; k < K; k++) {
+				val += dlt[k];
+				double d;
+				__builtin_memcpy(&d, &val, 8);
+				atomicAdd(&yw[c[k] & rmask], d * xv[k]);
+			}
+		}
+	}
+	for (; p0 < steps; p0 += K) {
+		uint32_t c[K];
+		int lb[K];
+#pragma unroll
+		for (int k = 0; k < K; k++) {
+			const bool ok = p0 + k < cnt;
+			const int at = ok ? crd + k * cstride : coords_at;
+			c[k] = CB2 ? (uint32_t) * (const uint16_t *) (pk + at) : lds_le_b32x(pk, at, cmask);
+			lb[k] = ok ? pk[la + k * lstride] : 0;
+		}
+		crd += K * cstride;
+		la += K * lstride;
+#pragma unroll
+		for (int k = 0; k < K; k++) {
+			const double xv = xs[c[k] >> rb];
+			const int len = lb[k] & 15, tz = (lb[k] >> 4) << 2;
+			val += stream_read<IL, 0>(pk, stream_at, cursor, len, 4 * L) << tz;
+			cursor += len;
+			double d;
+			__builtin_memcpy(&d, &val, 8);
+			if (p0 + k < cnt)
+				atomicAdd(&yw[c[k] & rmask], d * xv);
+		}
+	}
+	(void) lens_at;
+}
+
+template <int K, bool IL>
+__global__ void __launch_bounds__(kWave) div_slot_kernel(const uint8_t *__restrict__ bytes,
+		const int64_t *__restrict__ pk_off, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
+		long ngroups, int L, int buf_bytes, int xs_bytes, int acc_stride, int acc_copies, int xcd_k,
+		const double *__restrict__ x, double *__restrict__ y)
+{
+	extern __shared__ u32x4 smem[];
+	uint8_t *pk = (uint8_t *) smem;                    // buf_bytes (a multiple of 16, padding included)
+	double *xs = (double *) (pk + buf_bytes);          // xs_bytes: the staged x slots
+	double *yw = (double *) (pk + buf_bytes + xs_bytes); // acc_copies copies of the row accumulators
+	const int lane = lane_id();
+	double *ywl = yw + (lane & (acc_copies - 1)) * acc_stride;
+	const long g = xcd_logical_block(xcd_k);
+	if (g >= ngroups)
+		return;
+	const int p_end = grp_pk[g + 1];
+	int p = grp_pk[g];
+	const bool single = p_end - p == 1;
+	const long gr0 = grp_row[g], gr1 = grp_row[g + 1];
+	if (!single)
+		for (long r = gr0 + lane; r < gr1; r += kWave)
+			y[r] = 0.0;
+	for (; p < p_end; p++) {
+		// the packet copy, as in div_spmv_lds_kernel
+		const u32x4 *src = (const u32x4 *) (bytes + pk_off[p]);
+		const int n16 = (int) ((pk_off[p + 1] - pk_off[p] + 15) >> 4);
+		for (int j0 = 0; j0 < n16; j0 += kWave) {
+			if (j0 + lane < n16)
+				__builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *) (src + j0 + lane),
+						(__attribute__((address_space(3))) void *) (smem + j0), 16, 0, 2);
+		}
+		__builtin_amdgcn_s_waitcnt(0);
+		if (lane == 0)
+			smem[n16] = u32x4{0, 0, 0, 0};
+		wave_fence();
+		const SlotHeader z = slot_header(pk, L);
+		for (int r = lane; r < acc_copies * acc_stride; r += kWave)
+			yw[r] = 0.0;
+		slot_stage_x(pk, z, x, xs);
+		int stream_at = ((z.streams_at + 3) & ~3) + 4 * lane; // the lane's column of the interleaved streams
+		if (!IL) {
+			// this lane's value stream: exclusive scan of the lane sizes
+			const uint32_t *lsz = (const uint32_t *) (pk + z.lanes_at);
+			int my_size = lane < L ? (int) lsz[lane] : 0;
+			int incl = my_size;
+			for (int off = 1; off < kWave; off <<= 1) {
+				const int o = __shfl_up(incl, off, kWave);
+				if (lane >= off)
+					incl += o;
+			}
+			stream_at = z.streams_at + incl - my_size;
+		}
+		wave_fence(); // the staged x and the zeroed accumulators
+		const uint32_t *rfs = (const uint32_t *) (pk + z.rfs_at);
+		const uint32_t *per_rf = rfs + z.nrfs;
+		const uint32_t cmask = z.cb >= 4 ? 0xffffffffu : (1u << (8 * z.cb)) - 1;
+		const uint32_t rmask = (1u << z.rb) - 1;
+		int cursor = 0, ks = 0, cs = 0;
+		for (uint32_t gi = 0; gi < z.nrfs; gi++) {
+			const int rf = (int) __builtin_amdgcn_readfirstlane(rfs[gi]);
+			const int n = (int) __builtin_amdgcn_readfirstlane(per_rf[gi]);
+			const int dv = n / L, md = n % L, mul_c = dv * rf * L;
+			const int cnt = lane < L ? dv + (lane < md ? 1 : 0) : 0;
+			const int steps = dv + (md ? 1 : 0);
+			if (rf == 1) {
+				const int crd = z.coords_at + (cs + lane) * z.cb, la = z.lens_at + ks + lane;
+				if (z.cb == 2)
+					slot_rf1_group<K, true, IL>(pk, xs, z.coords_at, z.lens_at, crd, la, L * 2, L, stream_at, steps,
+							cnt, dv, L, 2, z.rb, ywl, cursor);
+				else
+					slot_rf1_group<K, false, IL>(pk, xs, z.coords_at, z.lens_at, crd, la, L * z.cb, L, stream_at,
+							steps, cnt, dv, L, z.cb, z.rb, ywl, cursor);
+				ks += n;
+				cs += n;
+				continue;
+			}
+			uint64_t val = 0;
+			for (int pp = 0; pp < cnt; pp++) {
+				const int lb = pk[z.lens_at + ks + pp * L + lane];
+				const int len = lb & 15, tz = (lb >> 4) << 2;
+				val += stream_read<IL, 0>(pk, stream_at, cursor, len, 4 * L) << tz;
+				cursor += len;
+				double d;
+				__builtin_memcpy(&d, &val, 8);
+				for (int t = 0; t < rf; t++) {
+					const int cpos = pp < dv ? (pp * rf + t) * L + lane : mul_c + lane * rf + t;
+					const uint32_t c = lds_le_b32x(pk, z.coords_at + (cs + cpos) * z.cb, cmask);
+					atomicAdd(&ywl[c & rmask], d * xs[c >> z.rb]);
+				}
+			}
+			ks += n;
+			cs += n * rf;
+		}
+		wave_fence();
+		if (single) {
+			for (long r = gr0 + lane; r < gr1; r += kWave) {
+				const long i = r - (long) z.row_min;
+				y[r] = i >= 0 && i < (long) z.nrows ? div_acc_sum(yw, acc_stride, acc_copies, (int) i) : 0.0;
+			}
+		} else {
+			for (uint32_t r = lane; r < z.nrows; r += kWave)
+				y[z.row_min + r] += div_acc_sum(yw, acc_stride, acc_copies, (int) r);
+		}
+		wave_fence(); // the packet buffer, slots and accumulators are reused
+	}
+}
+
+// Export of the decoded (row, col, value) sequence of the x-slot layout, in
+// packet order (the same sequence as div_export_kernel's on the reference
+// bytes). scratch: nnz int32, the slot -> column table of each packet at
+// its pk_nnz offset.
+__global__ void __launch_bounds__(kBlock) div_slot_export_kernel(const uint8_t *__restrict__ bytes,
+		const int64_t *__restrict__ pk_off, const int64_t *__restrict__ pk_nnz, const int32_t *__restrict__ grp_pk,
+		long ngroups, int L, int il, int32_t *__restrict__ scratch, int32_t *__restrict__ rows_out,
+		int32_t *__restrict__ cols_out, double *__restrict__ vals_out)
+{
+	const int lane = lane_id(), wv = threadIdx.x / kWave;
+	const long g = (long) blockIdx.x * kDivWavesPerBlock + wv;
+	if (g >= ngroups)
+		return;
+	for (int p = grp_pk[g]; p < grp_pk[g + 1]; p++) {
+		const uint8_t *pk = bytes + pk_off[p];
+		const uint32_t *h = (const uint32_t *) pk;
+		const uint32_t nv = h[0], nu = h[1], row_min = h[3], nslots = h[4];
+		const int rb = pk[20], cb = pk[22];
+		const uint32_t nrfs = h[6], nruns = h[7];
+		const long runs_at = spmvdiv::kSlotHeaderFixed + 4 * (long) L + 8 * (long) nrfs;
+		const long bits_at = (runs_at + 4 * (long) nruns + 7) & ~7L;
+		const long coords_at = bits_at + 8 * (((long) nslots + 63) >> 6);
+		const long lens_at = coords_at + (long) nv * cb;
+		const uint32_t *run_col = (const uint32_t *) (pk + runs_at);
+		const uint64_t *bits = (const uint64_t *) (pk + bits_at);
+		int32_t *slot_col = scratch + pk_nnz[p];
+		if (lane == 0) { // the table, slot by slot
+			long r = -1;
+			uint32_t start = 0;
+			for (uint32_t s = 0; s < nslots; s++) {
+				if (bits[s >> 6] >> (s & 63) & 1) {
+					r++;
+					start = s;
+				}
+				slot_col[s] = (int32_t) (run_col[r] + (s - start));
+			}
+		}
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+		__builtin_amdgcn_wave_barrier();
+		long my_size = lane < L ? (long) h[8 + lane] : 0;
+		long incl = my_size;
+		for (int off = 1; off < kWave; off <<= 1) {
+			long o = __shfl_up(incl, off, kWave);
+			if (lane >= off)
+				incl += o;
+		}
+		const uint32_t *rfs = (const uint32_t *) (pk + spmvdiv::kSlotHeaderFixed + 4 * L);
+		const uint32_t *per_rf = rfs + nrfs;
+		const uint8_t *lens = pk + lens_at;
+		const long sbase = il ? (lens_at + nu + 3) & ~3L : lens_at + nu + (incl - my_size);
+		const uint64_t rmask = rb ? (1ull << rb) - 1 : 0;
+		const long nnz0 = pk_nnz[p];
+		long cursor = 0, ks = 0, cs = 0;
+		for (uint32_t gi = 0; gi < nrfs; gi++) {
+			const long rf = rfs[gi], n = per_rf[gi];
+			const long dv = n / L, md = n % L, mul_c = dv * rf * L;
+			if (lane < L) {
+				uint64_t val = 0;
+				const long cnt = dv + (lane < md ? 1 : 0);
+				for (long pp = 0; pp < cnt; pp++) {
+					const uint8_t lb = lens[ks + pp * L + lane];
+					const int len = lb & 15, tz = (lb >> 4) << 2;
+					if (len) {
+						val += stream_le(pk, sbase, lane, cursor, len, il, 4 * L) << tz;
+						cursor += len;
+					}
+					double d;
+					memcpy(&d, &val, 8);
+					for (long t = 0; t < rf; t++) {
+						const long cpos = pp < dv ? (pp * rf + t) * L + lane : mul_c + lane * rf + t;
+						const uint64_t c = load_le(pk + coords_at + (cs + cpos) * cb, cb);
+						const long o = nnz0 + cs + cpos;
+						rows_out[o] = (int32_t) (row_min + (uint32_t) (c & rmask));
+						cols_out[o] = slot_col[c >> rb];
+						vals_out[o] = d;
+					}
+				}
+			}
+			ks += n;
+			cs += n * rf;
+		}
 	}
 }
 
@@ -620,26 +1025,12 @@ int div_convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col, co
 	if (spmvdiv::encode(h_rp, h_col, h_val, f->m, f->n, f->nnz, T, L, P, &st, &err))
 		return set_error(SPMVHIP_EFORMAT, "div_rf: %s (the reference rejects this matrix too)", err);
 	const long np = st.npackets(), ng = st.ngroups();
-	// 16-byte aligned packet starts on the device
-	std::vector<int64_t> doff(np + 1);
-	int64_t at = 0;
-	for (long p = 0; p < np; p++) {
-		doff[p] = at;
-		at += (st.pk_off[p + 1] - st.pk_off[p] + 15) / 16 * 16;
-	}
-	doff[np] = at;
-	std::vector<uint8_t> dev(at + 16, 0); // +16: the decoder reads whole aligned 8-byte words
-	for (long p = 0; p < np; p++)
-		memcpy(&dev[doff[p]], &st.bytes[st.pk_off[p]], st.pk_off[p + 1] - st.pk_off[p]);
+	if (f->p.div_layout < 0 || f->p.div_layout > 4)
+		return set_error(SPMVHIP_EINVAL, "div_rf: div_layout %d (0 automatic, 1 reference bytes, 2 x slots, "
+				"3 interleaved lane streams, 4 x slots and interleaved lane streams)", f->p.div_layout);
 	f->div_npackets = np;
 	f->div_ngroups = ng;
-	f->div_bytes = (long) st.bytes.size();
-	int64_t max_pk = 0;
-	for (long p = 0; p < np; p++)
-		max_pk = std::max<int64_t>(max_pk, doff[p + 1] - doff[p]);
-	// the staged decoder keeps a packet (16-byte rounded, + 16 of zero padding)
-	// and one accumulator per row of the packet in LDS: used while that fits
-	// 64 KiB. Header word 2 is the packet's row count (at most 255).
+	// the most rows of any packet (header word 2, at most 255)
 	uint32_t max_rows = 1;
 	for (long p = 0; p < np; p++) {
 		uint32_t nr;
@@ -656,7 +1047,70 @@ int div_convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col, co
 	}
 	f->p.div_acc_copies = nc;
 	f->div_acc_bytes = (int) ((nc * f->div_acc_stride * sizeof(double) + 15) / 16 * 16);
-	f->div_buf_bytes = max_pk + 16 + f->div_acc_bytes <= kDivLdsMax ? (int) (max_pk + 16) : 0;
+	// the device layout: the requested one, or (automatic) the first of
+	// kDivAutoLayouts whose packet (+ 16 of zero padding, + 2 rows of stream
+	// slack when interleaved), staged x slots and row accumulators fit the LDS
+	// of one workgroup; the reference bytes when none does
+	std::vector<uint8_t> dev;
+	std::vector<int64_t> doff;
+	int64_t max_pk = 0;
+	f->div_slots = false;
+	f->div_il = false;
+	f->div_xs_bytes = 0;
+	const int want = f->p.div_layout;
+	const int *tries = want == 0 ? kDivAutoLayouts : &want;
+	const int ntries = want == 0 ? (int) (sizeof(kDivAutoLayouts) / sizeof(int)) : 1;
+	bool placed = false;
+	for (int t = 0; t < ntries && np > 0 && !placed; t++) {
+		const int lay = tries[t];
+		const bool slots = lay == 2 || lay == 4, il = lay == 3 || lay == 4;
+		spmvdiv::DeviceLayout dl;
+		if (spmvdiv::to_device_layout(st, slots, il, &dl, &err)) {
+			if (want != 0)
+				return set_error(SPMVHIP_EFORMAT, "div_rf: %s", err);
+			continue;
+		}
+		const long xs_bytes = slots ? ((long) dl.max_slots * 8 + 15) / 16 * 16 : 0;
+		const long need = dl.max_packet + 16 + (il ? 8L * L : 0) + xs_bytes + f->div_acc_bytes;
+		if (lay != 1 && need > kDivLdsMax) {
+			if (want != 0)
+				return set_error(SPMVHIP_EINVAL, "div_rf: layout %d needs %ld bytes of LDS per packet (at most %ld)",
+						lay, need, kDivLdsMax);
+			continue;
+		}
+		f->div_slots = slots;
+		f->div_il = il;
+		f->div_xs_bytes = (int) xs_bytes;
+		max_pk = dl.max_packet + (il ? 8L * L : 0);
+		dev.swap(dl.bytes);
+		doff.swap(dl.pk_off);
+		placed = true;
+	}
+	if (!placed) {
+		// the reference's packet bytes, each starting at a 16-byte aligned offset
+		doff.assign(np + 1, 0);
+		int64_t at = 0;
+		for (long p = 0; p < np; p++) {
+			doff[p] = at;
+			at += (st.pk_off[p + 1] - st.pk_off[p] + 15) / 16 * 16;
+		}
+		doff[np] = at;
+		dev.assign(at + 16, 0); // +16: the decoder reads whole aligned 8-byte words
+		for (long p = 0; p < np; p++)
+			memcpy(&dev[doff[p]], &st.bytes[st.pk_off[p]], st.pk_off[p + 1] - st.pk_off[p]);
+		for (long p = 0; p < np; p++)
+			max_pk = std::max<int64_t>(max_pk, doff[p + 1] - doff[p]);
+	}
+	f->div_bytes = (long) doff[np > 0 ? np : 0];
+	// the staged decoders keep a packet (16-byte rounded, + 16 of zero padding,
+	// + stream slack), the x slots and one accumulator per row of the packet in
+	// LDS: used while that fits 64 KiB (else the reference bytes are decoded
+	// from global memory)
+	f->div_buf_bytes = max_pk + 16 + f->div_xs_bytes + f->div_acc_bytes <= kDivLdsMax ? (int) (max_pk + 16) : 0;
+	if (dev.empty())
+		dev.assign(16, 0);
+	if (doff.empty())
+		doff.assign(1, 0);
 	SPMVHIP_TRY(f->div_stream.alloc(dev.size()));
 	SPMVHIP_TRY(f->div_pk_off.alloc((np + 1) * sizeof(int64_t)));
 	SPMVHIP_TRY(f->div_pk_nnz.alloc((np + 1) * sizeof(int64_t)));
@@ -686,22 +1140,50 @@ int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 		SPMVHIP_CHECK(hipGetLastError());
 		return SPMVHIP_OK;
 	}
-	if (f->div_buf_bytes > 0) {
+	if (f->div_slots) {
+		const size_t lds = (size_t) f->div_buf_bytes + (size_t) f->div_xs_bytes + (size_t) f->div_acc_bytes;
+		const int xk = xcd_chunk(f->p.xcd_remap);
+		const unsigned grid = xcd_padded_grid((unsigned) f->div_ngroups, xk);
+#define LAUNCH_SLOT(K, IL)                                                                                        \
+	div_slot_kernel<K, IL><<<grid, kWave, lds, s>>>(f->div_stream.as<uint8_t>(), f->div_pk_off.as<int64_t>(),  \
+			f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(), f->div_ngroups, f->div_lanes,     \
+			f->div_buf_bytes, f->div_xs_bytes, f->div_acc_stride, f->p.div_acc_copies, xk, x, y)
+#define LAUNCH_SLOT_K(IL)          \
+	if (f->p.unroll >= 16)     \
+		LAUNCH_SLOT(16, IL); \
+	else if (f->p.unroll >= 8) \
+		LAUNCH_SLOT(8, IL);  \
+	else                       \
+		LAUNCH_SLOT(4, IL)
+		if (f->div_il) {
+			LAUNCH_SLOT_K(true);
+		} else {
+			LAUNCH_SLOT_K(false);
+		}
+#undef LAUNCH_SLOT_K
+#undef LAUNCH_SLOT
+	} else if (f->div_buf_bytes > 0) {
 		const size_t lds = (size_t) f->div_buf_bytes + (size_t) f->div_acc_bytes;
 		const int xk = xcd_chunk(f->p.xcd_remap);
 		const unsigned grid = xcd_padded_grid((unsigned) f->div_ngroups, xk);
-		if (f->p.unroll >= 16)
-			div_spmv_lds_kernel<16><<<grid, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
-					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
-					f->div_ngroups, f->div_lanes, f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, xk, x, y);
-		else if (f->p.unroll >= 8)
-			div_spmv_lds_kernel<8><<<grid, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
-					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
-					f->div_ngroups, f->div_lanes, f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, xk, x, y);
-		else
-			div_spmv_lds_kernel<4><<<grid, kWave, lds, s>>>(f->div_stream.as<uint8_t>(),
-					f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
-					f->div_ngroups, f->div_lanes, f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, xk, x, y);
+#define LAUNCH_LDS(K, IL)                                                                                       \
+	div_spmv_lds_kernel<K, IL><<<grid, kWave, lds, s>>>(f->div_stream.as<uint8_t>(), f->div_pk_off.as<int64_t>(), \
+			f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(), f->div_ngroups, f->div_lanes,      \
+			f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, xk, x, y)
+#define LAUNCH_LDS_K(IL)           \
+	if (f->p.unroll >= 16)     \
+		LAUNCH_LDS(16, IL);  \
+	else if (f->p.unroll >= 8) \
+		LAUNCH_LDS(8, IL);   \
+	else                       \
+		LAUNCH_LDS(4, IL)
+		if (f->div_il) {
+			LAUNCH_LDS_K(true);
+		} else {
+			LAUNCH_LDS_K(false);
+		}
+#undef LAUNCH_LDS_K
+#undef LAUNCH_LDS
 	} else {
 		div_spmv_kernel<8><<<grid_for(f->div_ngroups, kDivWavesPerBlock), kBlock, 0, s>>>(f->div_stream.as<uint8_t>(),
 				f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
@@ -724,14 +1206,22 @@ int div_export_decoded(const spmvhip_format *f, int which, void *buf, size_t *by
 		*bytes = 0;
 		return SPMVHIP_OK;
 	}
-	DevBuf rows, cols, vals;
+	DevBuf rows, cols, vals, scratch;
 	SPMVHIP_TRY(rows.alloc(f->nnz * sizeof(int32_t)));
 	SPMVHIP_TRY(cols.alloc(f->nnz * sizeof(int32_t)));
 	SPMVHIP_TRY(vals.alloc(f->nnz * sizeof(double)));
-	div_export_kernel<<<grid_for(f->div_ngroups, kDivWavesPerBlock), kBlock, 0, f->stream>>>(
-			f->div_stream.as<uint8_t>(), f->div_pk_off.as<int64_t>(), f->div_pk_nnz.as<int64_t>(),
-			f->div_groups.as<int32_t>(), f->div_ngroups, f->div_lanes, rows.as<int32_t>(), cols.as<int32_t>(),
-			vals.as<double>());
+	if (f->div_slots) {
+		SPMVHIP_TRY(scratch.alloc(f->nnz * sizeof(int32_t)));
+		div_slot_export_kernel<<<grid_for(f->div_ngroups, kDivWavesPerBlock), kBlock, 0, f->stream>>>(
+				f->div_stream.as<uint8_t>(), f->div_pk_off.as<int64_t>(), f->div_pk_nnz.as<int64_t>(),
+				f->div_groups.as<int32_t>(), f->div_ngroups, f->div_lanes, f->div_il ? 1 : 0, scratch.as<int32_t>(),
+				rows.as<int32_t>(), cols.as<int32_t>(), vals.as<double>());
+	} else {
+		div_export_kernel<<<grid_for(f->div_ngroups, kDivWavesPerBlock), kBlock, 0, f->stream>>>(
+				f->div_stream.as<uint8_t>(), f->div_pk_off.as<int64_t>(), f->div_pk_nnz.as<int64_t>(),
+				f->div_groups.as<int32_t>(), f->div_ngroups, f->div_lanes, f->div_il ? 1 : 0, rows.as<int32_t>(),
+				cols.as<int32_t>(), vals.as<double>());
+	}
 	SPMVHIP_CHECK(hipGetLastError());
 	SPMVHIP_CHECK(hipStreamSynchronize(f->stream));
 	const void *src = which == 6 ? rows.p : which == 7 ? cols.p : vals.p;
@@ -767,5 +1257,37 @@ extern "C" int spmvhip_div_encode(const int32_t *row_ptr, const int32_t *col_ind
 	*nbytes = st.bytes.size();
 	*npackets = st.npackets();
 	*ngroups = st.ngroups();
+	return SPMVHIP_OK;
+}
+
+extern "C" int spmvhip_div_device_encode(const int32_t *row_ptr, const int32_t *col_ind, const double *values,
+		long m, long n, long nnz, int T, int L, long P, int layout, uint8_t *bytes, size_t *nbytes, int64_t *pk_off,
+		long *npackets)
+{
+	using spmvhip::set_error;
+	if (!row_ptr || (nnz > 0 && (!col_ind || !values)) || !nbytes || !npackets)
+		return set_error(SPMVHIP_EINVAL, "NULL argument");
+	if (T < 1 || L < 1 || P < 1 || m < 0)
+		return set_error(SPMVHIP_EINVAL, "T, L, P must be positive");
+	if (layout < 1 || layout > 4)
+		return set_error(SPMVHIP_EINVAL, "layout %d (1..4)", layout);
+	spmvdiv::Stream st;
+	const char *err = nullptr;
+	if (spmvdiv::encode(row_ptr, col_ind, values, m, n, nnz, T, L, P, &st, &err))
+		return set_error(SPMVHIP_EFORMAT, "div_rf: %s", err);
+	spmvdiv::DeviceLayout dl;
+	if (spmvdiv::to_device_layout(st, layout == 2 || layout == 4, layout == 3 || layout == 4, &dl, &err))
+		return set_error(SPMVHIP_EFORMAT, "div_rf: %s", err);
+	const size_t total = (size_t) dl.pk_off.back();
+	if (bytes) {
+		if (*nbytes < total || *npackets < st.npackets())
+			return set_error(SPMVHIP_EINVAL, "output buffers too small");
+		if (total)
+			memcpy(bytes, dl.bytes.data(), total);
+		if (pk_off)
+			memcpy(pk_off, dl.pk_off.data(), dl.pk_off.size() * sizeof(int64_t));
+	}
+	*nbytes = total;
+	*npackets = st.npackets();
 	return SPMVHIP_OK;
 }

@@ -49,6 +49,9 @@ struct spmvhip_format {
 	int div_buf_bytes = 0;        // LDS packet buffer of the staged decoder: largest packet + 16, or 0 if too large
 	int div_acc_bytes = 2048;     // LDS row accumulators of the staged decoder (all copies)
 	int div_acc_stride = 255;     // rows per accumulator copy: the most rows of any packet, odd
+	bool div_slots = false;       // packets in the x-slot layout (div_codec.hpp)
+	bool div_il = false;          // lane value streams interleaved by dwords (div_codec.hpp)
+	int div_xs_bytes = 0;         // LDS of the staged x slots: the most slots of any packet x 8, 16-rounded
 
 	// host-pointer contract
 	const double *bound_x = nullptr;

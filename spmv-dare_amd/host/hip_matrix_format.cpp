@@ -15,7 +15,7 @@
 //   SELLCS_C, SELLCS_SIGMA          SELL-C-sigma slice height / sort window
 //   CSRV_BLOCK_NNZ, CSRV_LANES, CSRV_SPLIT_NNZ, CSRV_NARROW_LOADS   CSR-vector row-block tuning
 //   SPMV_XCD_REMAP, SELL_UNROLL, SELL_PIPELINE, SELL_X_WINDOW, SELL_Y_MODE
-//   DIV_THREADS, DIV_LANES, CSRCV_NUM_PACKET_VALS   DIV-RF codec parameters
+//   DIV_THREADS, DIV_LANES, CSRCV_NUM_PACKET_VALS, DIV_LAYOUT   DIV-RF codec / device layout
 
 #include <cstdio>
 #include <cstdlib>
@@ -116,6 +116,7 @@ struct Matrix_Format *csr_to_format(INT_T *row_ptr, INT_T *col_ind, ValueTypeRef
 	p.div_lanes = (int) env_long("DIV_LANES", 0);
 	p.div_acc_copies = (int) env_long("DIV_ACC_COPIES", 0);
 	p.narrow_loads = (int) env_long("CSRV_NARROW_LOADS", 0);
+	p.div_layout = (int) env_long("DIV_LAYOUT", 0);
 	p.div_packet_vals = env_long("CSRCV_NUM_PACKET_VALS", 0);
 
 	HipMatrixFormat *f = new HipMatrixFormat(m, n, nnz);

@@ -31,7 +31,7 @@ class Params(ctypes.Structure):
                 ("div_threads", ctypes.c_int), ("div_lanes", ctypes.c_int),
                 ("div_packet_vals", ctypes.c_long), ("pipeline", ctypes.c_int),
                 ("y_mode", ctypes.c_int), ("div_acc_copies", ctypes.c_int), ("narrow_loads", ctypes.c_int),
-                ("reserved", ctypes.c_int * 4)]
+                ("div_layout", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 class Info(ctypes.Structure):
@@ -98,6 +98,9 @@ _sig = {
     "spmvhip_div_encode": ([_vp, _vp, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_int,
                             ctypes.c_int, ctypes.c_long, _vp, ctypes.POINTER(ctypes.c_size_t), _vp,
                             ctypes.POINTER(ctypes.c_long), _vp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+    "spmvhip_div_device_encode": ([_vp, _vp, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_long, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_size_t),
+                                   _vp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
 }
 for _name, (_args, _res) in _sig.items():
     _f = getattr(_lib, _name)
@@ -441,3 +444,23 @@ def statistics_print_labels():
     buf = ctypes.create_string_buffer(512)
     _lib.spmvhip_statistics_print_labels(buf, 512)
     return buf.value.decode()
+
+
+def div_device_encode(row_ptr, col_ind, values, n, T=1, L=64, P=16384, layout=2):
+    """A device layout of the DIV-RF stream (layout: 1 reference bytes, 2 x
+    slots, 3 interleaved lane streams, 4 both; no device needed): returns
+    (bytes, packet byte offsets)."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    ci = np.ascontiguousarray(col_ind, dtype=np.int32)
+    va = np.ascontiguousarray(values, dtype=np.float64)
+    m, nnz = rp.size - 1, ci.size
+    nb, npk = ctypes.c_size_t(), ctypes.c_long()
+    _check(_lib.spmvhip_div_device_encode(rp.ctypes.data, ci.ctypes.data, va.ctypes.data, m, n, nnz, T, L, P,
+                                          layout, None, ctypes.byref(nb), None, ctypes.byref(npk)),
+           "div_device_encode")
+    b = np.empty(max(nb.value, 1), dtype=np.uint8)
+    off = np.empty(npk.value + 1, dtype=np.int64)
+    _check(_lib.spmvhip_div_device_encode(rp.ctypes.data, ci.ctypes.data, va.ctypes.data, m, n, nnz, T, L, P,
+                                          layout, b.ctypes.data, ctypes.byref(nb), off.ctypes.data,
+                                          ctypes.byref(npk)), "div_device_encode")
+    return b[:nb.value], off

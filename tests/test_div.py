@@ -83,3 +83,95 @@ def test_roundtrip_repeated_values_and_empty_rows():
     va[::7] = rng.uniform(-1, 1, va[::7].size)
     for T, L, P in [(1, 4, 16384), (3, 64, 200), (5, 13, 777)]:
         roundtrip(rp, ci, va, n, T, L, P)
+
+
+def _u32(b, at):
+    return int(np.frombuffer(b[at:at + 4].tobytes(), dtype=np.uint32)[0])
+
+
+def _le(b, at, nbytes):
+    return int.from_bytes(b[at:at + nbytes].tobytes(), "little")
+
+
+def _streams(pk, at, lane_sizes, L, il):
+    """The L lane streams of a device packet's stream section (from byte
+    `at`), one bytes object per lane: contiguous, or de-interleaved from rows
+    of L dwords."""
+    out, base = [], at
+    for l in range(L):
+        sz = int(lane_sizes[l])
+        if il:
+            idx = at + (np.arange(sz) >> 2) * 4 * L + 4 * l + (np.arange(sz) & 3)
+            out.append(pk[idx].tobytes())
+        else:
+            out.append(pk[base:base + sz].tobytes())
+            base += sz
+    return out
+
+
+@pytest.mark.parametrize("layout", [1, 2, 3, 4])
+@pytest.mark.parametrize("name,T,L,P", [("test_txt", 4, 4, 512), ("cfg1", 2, 64, 1024), ("batch_03", 3, 32, 300),
+                                        ("dense_rows", 1, 64, 16384), ("tall", 8, 64, 97)])
+def test_device_layout_decodes_to_reference_packets(layout, name, T, L, P):
+    """Every device layout of every packet against the reference packet:
+    x slots (layouts 2, 4) are the packet's distinct columns in ascending
+    order (runs maximal, one start bit per run) and every coordinate maps back
+    to the (row, column) the reference packet holds at the same position;
+    interleaved streams (3, 4) de-interleave to the reference lane streams;
+    lengths and everything else are the reference's bytes. The decode below
+    is restated here, not taken from the library."""
+    slots, il = layout in (2, 4), layout in (3, 4)
+    a = gen.generate(gen.GenParams.parse(GEN[name]["line"]), stats=False)
+    b, off, grp = hip.div_encode(a.row_ptr, a.col_ind, a.values, a.n, T=T, L=L, P=P)
+    r, c, v = pyoracle.div_decode(b, L, a.nnz)
+    sb, soff = hip.div_device_encode(a.row_ptr, a.col_ind, a.values, a.n, T=T, L=L, P=P, layout=layout)
+    assert soff.size == off.size and (soff % 16 == 0).all()
+    k0 = 0
+    for p in range(off.size - 1):
+        ref = b[off[p]:off[p + 1]]
+        pk = sb[soff[p]:soff[p + 1]]
+        nv, nu, nrfs = _u32(ref, 0), _u32(ref, 4), _u32(ref, 24)
+        ref_cb = (int(ref[20]) + int(ref[21]) + 7) // 8
+        ref_lens_at = 28 + 4 * L + 8 * nrfs + nv * ref_cb
+        lane_sizes = np.frombuffer(ref[28:28 + 4 * L].tobytes(), dtype=np.uint32)
+        ref_streams = _streams(ref, ref_lens_at + nu, lane_sizes, L, False)
+        if not slots:
+            assert int(pk[22]) == int(il)
+            np.testing.assert_array_equal(np.delete(pk[:ref_lens_at + nu], 22), np.delete(ref[:ref_lens_at + nu], 22))
+            assert _streams(pk, (ref_lens_at + nu + 3) // 4 * 4 if il else ref_lens_at + nu, lane_sizes, L,
+                            il) == ref_streams
+            k0 += nv
+            continue
+        nv2, nu2, nrows, row_min, nslots = (_u32(pk, 4 * i) for i in range(5))
+        rb, sbits, cb, il_flag = int(pk[20]), int(pk[21]), int(pk[22]), int(pk[23])
+        nrfs2, nruns = _u32(pk, 24), _u32(pk, 28)
+        assert (nv2, nu2, nrows, row_min, nrfs2, il_flag) == tuple(_u32(ref, 4 * i) for i in range(4)) + (nrfs,
+                                                                                                              int(il))
+        np.testing.assert_array_equal(pk[32:32 + 4 * L], ref[28:28 + 4 * L])            # lane sizes
+        np.testing.assert_array_equal(pk[32 + 4 * L:32 + 4 * L + 8 * nrfs],
+                                      ref[28 + 4 * L:28 + 4 * L + 8 * nrfs])                # rfs, unique per rf
+        runs_at = 32 + 4 * L + 8 * nrfs
+        bits_at = (runs_at + 4 * nruns + 7) // 8 * 8
+        coords_at = bits_at + 8 * ((nslots + 63) // 64)
+        run_col = np.frombuffer(pk[runs_at:runs_at + 4 * nruns].tobytes(), dtype=np.uint32)
+        words = np.frombuffer(pk[bits_at:coords_at].tobytes(), dtype=np.uint64)
+        starts = [s for s in range(nslots) if (int(words[s // 64]) >> (s % 64)) & 1]
+        assert starts[0] == 0 and len(starts) == nruns
+        slot_col = np.empty(nslots, dtype=np.int64)
+        bounds = starts + [nslots]
+        for i in range(nruns):
+            slot_col[bounds[i]:bounds[i + 1]] = int(run_col[i]) + np.arange(bounds[i + 1] - bounds[i])
+        cols = np.asarray(c[k0:k0 + nv], dtype=np.int64)
+        np.testing.assert_array_equal(slot_col, np.unique(cols))                           # slots = distinct columns
+        assert all(run_col[i] > slot_col[bounds[i] - 1] + 1 for i in range(1, nruns))     # runs are maximal
+        assert sbits == (int(nslots - 1).bit_length() if nslots > 1 else 0)
+        assert cb == max(1, (rb + sbits + 7) // 8) and rb == (int(nrows - 1).bit_length() if nrows > 1 else 0)
+        for k in range(nv):
+            cc = _le(pk, coords_at + k * cb, cb)
+            assert row_min + (cc & ((1 << rb) - 1)) == r[k0 + k]
+            assert slot_col[cc >> rb] == c[k0 + k]
+        lens_at = coords_at + nv * cb
+        np.testing.assert_array_equal(pk[lens_at:lens_at + nu], ref[ref_lens_at:ref_lens_at + nu])
+        assert _streams(pk, (lens_at + nu + 3) // 4 * 4 if il else lens_at + nu, lane_sizes, L, il) == ref_streams
+        k0 += nv
+    assert k0 == a.nnz

@@ -113,3 +113,34 @@ def test_div_lanes_above_wave_rejected():
     with pytest.raises(hip.SpmvHipError) as e:
         hip.csr_to_format(rp, ci, va, rp.size - 1, n, ci.size, kind="div_rf", div_lanes=128)
     assert e.value.code == -1
+
+
+@pytest.mark.parametrize("name", ["test_txt", "cfg1", "ragged_4097", "dense_rows", "huge_rows", "wide_diag_2m"])
+@pytest.mark.parametrize("layout", [1, 2, 3, 4])
+@pytest.mark.parametrize("T,L,P,unroll", [(2, 64, 1024, 8), (3, 32, 300, 4), (1, 64, 512, 16)])
+def test_div_device_layouts(name, layout, T, L, P, unroll):
+    """Every device layout (1 reference bytes, 2 x slots, 3 interleaved lane
+    streams, 4 both): device decode bit-identical to the oracle's, y within
+    1e-13 of csr_naive."""
+    if name in WIDE:
+        a = gen.generate(gen.GenParams.parse(WIDE[name]), stats=False)
+        rp, ci, va, n = a.row_ptr, a.col_ind, a.values, a.n
+    else:
+        rp, ci, va, n = matrix(name)
+    check_div(rp, ci, va, n, T, L, P, unroll, name, div_layout=layout)
+
+
+@pytest.mark.parametrize("name", ["cfg1", "dense_rows"])
+@pytest.mark.parametrize("layout", [2, 3, 4])
+def test_div_device_layouts_repeated_values(name, layout):
+    """Replication-factor groups above 1 (values rounded to halves) in every
+    device layout: the decoders' general path."""
+    rp, ci, va, n = matrix(name)
+    check_div(rp, ci, np.round(va * 2) / 2, n, 2, 64, 1024, 8, name + "_halves", div_layout=layout)
+
+
+def test_div_layout_rejected():
+    rp, ci, va, n = matrix("test_txt")
+    for bad in (-1, 5):
+        with pytest.raises(hip.SpmvHipError):
+            hip.csr_to_format(rp, ci, va, rp.size - 1, n, ci.size, kind="div_rf", div_layout=bad)
