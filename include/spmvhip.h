@@ -200,9 +200,12 @@ void spmvhip_destroy(spmvhip_format *f);
  * next ticket before it runs the current unit; long-row segments are
  * completed by a second kernel of the same sequence. With use_graph the
  * sequence (counter reset + kernel + segment finalize) is captured once into
- * a hipGraph and replayed. Members must be CSR_VEC (block_nnz <= 512), SELL or
- * SELL_C_S handles, must outlive the batch and must not run elsewhere
- * meanwhile. tile_units: consecutive units per ticket (0 = 1, at most 65536). */
+ * a hipGraph and replayed. DIV_RF members (LDS-staged decoder, div_layout 1 or
+ * 3) are decoded by a second persistent kernel of the same sequence that draws
+ * all DIV members' packet groups by ticket in the same way. Members must be
+ * CSR_VEC (block_nnz <= 512), SELL, SELL_C_S or DIV_RF handles, must outlive
+ * the batch and must not run elsewhere meanwhile. tile_units: consecutive
+ * units per ticket (0 = 1, at most 65536). */
 typedef struct spmvhip_batch spmvhip_batch;
 int spmvhip_batch_create(int tile_units, spmvhip_batch **out);
 /* Ticket counters (before finalize): a power of two, 1..1024 (default 256),

@@ -231,12 +231,19 @@ struct spmvhip_batch {
 	long nunits = 0;
 	std::vector<spmvhip::SplitRef> host_splits;
 	spmvhip::DevBuf mats, units, counter, splits;
+	// DIV_RF members: their packet groups, drawn by a second persistent kernel
+	std::vector<spmvhip::BatchDivMat> host_div;
+	std::vector<spmvhip::BatchDivUnit> host_div_units;
+	spmvhip::DevBuf div_mats, div_units, div_counter;
+	int div_buf = 0, div_acc = 0, div_grid = 0, div_nc = 1;
 	hipStream_t stream = nullptr;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	int grid = 0;
 	bool finalized = false;
 	long total_nnz = 0;
+	std::vector<int> div_fmts;           // indices into fmts of the DIV members
+	std::vector<std::pair<uint64_t, uint64_t>> div_xy; // their x and y
 };
 
 using namespace spmvhip;
@@ -261,6 +268,10 @@ int batch_launch(spmvhip_batch *b, hipStream_t s)
 				b->mats.as<BatchMat>(), b->splits.as<SplitRef>(), (long) b->host_splits.size());
 		SPMVHIP_CHECK(hipGetLastError());
 	}
+	if (!b->host_div_units.empty())
+		SPMVHIP_TRY(div_batch_launch(b->div_mats.as<BatchDivMat>(), b->div_units.as<BatchDivUnit>(),
+				(int) b->host_div_units.size(), b->div_nc, kCounterStride, b->div_counter.as<int>(), b->div_grid,
+				b->div_buf, b->div_acc, s));
 	return SPMVHIP_OK;
 }
 
@@ -364,8 +375,26 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 		M.order = f->kind == SPMVHIP_SELL_C_S ? ptr_bits(f->row_order.p) : 0;
 		M.lanes = f->lanes; // lanes per row T (>= 1) of the slice layout
 		units = f->nslices;
+	} else if (f->kind == SPMVHIP_DIV_RF) {
+		BatchDivMat D{};
+		int buf = 0, acc = 0;
+		SPMVHIP_TRY(div_batch_member(f, d_x, d_y, &D, &buf, &acc));
+		b->div_xy.push_back({ptr_bits(d_x), ptr_bits(d_y)});
+		if (b->host_div.size() >= (size_t) INT32_MAX || f->div_ngroups > INT32_MAX ||
+				(long) b->host_div_units.size() + f->div_ngroups > INT32_MAX - (long) kCountersMax)
+			return set_error(SPMVHIP_EINVAL, "batch: too many DIV packet groups");
+		const int di = (int) b->host_div.size();
+		b->host_div.push_back(D);
+		for (long g = 0; g < f->div_ngroups; g++)
+			b->host_div_units.push_back({di, (int) g});
+		b->div_buf = std::max(b->div_buf, buf);
+		b->div_acc = std::max(b->div_acc, acc);
+		b->fmts.push_back(const_cast<spmvhip_format *>(f));
+		b->div_fmts.push_back((int) b->fmts.size() - 1);
+		b->total_nnz += f->nnz;
+		return SPMVHIP_OK;
 	} else {
-		return set_error(SPMVHIP_EINVAL, "batch: format %s is not batchable (csr_vec, sell, sell_c_s are)",
+		return set_error(SPMVHIP_EINVAL, "batch: format %s is not batchable (csr_vec, sell, sell_c_s, div_rf are)",
 				f->name);
 	}
 	if (units > INT32_MAX || nsegs > INT32_MAX || b->host_mats.size() >= (size_t) kMatMask)
@@ -477,6 +506,23 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 		SPMVHIP_CHECK(hipMemcpy(b->splits.p, b->host_splits.data(), b->host_splits.size() * sizeof(SplitRef),
 				hipMemcpyHostToDevice));
 	}
+	if (!b->host_div_units.empty()) {
+		int per_cu = 0;
+		SPMVHIP_TRY(div_batch_occupancy(b->div_buf + b->div_acc, &per_cu));
+		if (per_cu < 1)
+			return set_error(SPMVHIP_EINVAL, "batch: DIV members need %d bytes of LDS", b->div_buf + b->div_acc);
+		b->div_grid = cus * per_cu;
+		b->div_nc = 1;
+		while (b->div_nc * 2 <= std::min(b->queues, b->div_grid))
+			b->div_nc *= 2;
+		SPMVHIP_TRY(b->div_mats.alloc(b->host_div.size() * sizeof(BatchDivMat)));
+		SPMVHIP_TRY(b->div_units.alloc(b->host_div_units.size() * sizeof(BatchDivUnit)));
+		SPMVHIP_TRY(b->div_counter.alloc((size_t) kCountersMax * kCounterStride * sizeof(int)));
+		SPMVHIP_CHECK(hipMemcpy(b->div_mats.p, b->host_div.data(), b->host_div.size() * sizeof(BatchDivMat),
+				hipMemcpyHostToDevice));
+		SPMVHIP_CHECK(hipMemcpy(b->div_units.p, b->host_div_units.data(),
+				b->host_div_units.size() * sizeof(BatchDivUnit), hipMemcpyHostToDevice));
+	}
 	b->finalized = true;
 	if (use_graph) {
 		SPMVHIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -519,9 +565,21 @@ int spmvhip_batch_time(spmvhip_batch *b, int per_matrix, int warmup, int iters, 
 	auto once = [&]() -> int {
 		if (!per_matrix)
 			return spmvhip_batch_run(b);
-		for (size_t i = 0; i < b->fmts.size(); i++)
-			SPMVHIP_TRY(spmvhip_spmv_device(b->fmts[i], (const double *) (uintptr_t) b->host_mats[i].x,
-					(double *) (uintptr_t) b->host_mats[i].y, b->stream));
+		size_t mi = 0, di = 0;
+		for (size_t i = 0; i < b->fmts.size(); i++) {
+			uint64_t x, y;
+			if (di < b->div_fmts.size() && b->div_fmts[di] == (int) i) {
+				x = b->div_xy[di].first;
+				y = b->div_xy[di].second;
+				di++;
+			} else {
+				x = b->host_mats[mi].x;
+				y = b->host_mats[mi].y;
+				mi++;
+			}
+			SPMVHIP_TRY(spmvhip_spmv_device(b->fmts[i], (const double *) (uintptr_t) x, (double *) (uintptr_t) y,
+					b->stream));
+		}
 		return SPMVHIP_OK;
 	};
 	for (int i = 0; i < warmup; i++)
@@ -556,9 +614,9 @@ int spmvhip_batch_info(const spmvhip_batch *b, long *nmats, long *nitems, int *g
 	if (!b)
 		return set_error(SPMVHIP_EINVAL, "NULL batch");
 	if (nmats)
-		*nmats = (long) b->host_mats.size();
+		*nmats = (long) b->fmts.size();
 	if (nitems)
-		*nitems = b->nunits;
+		*nitems = b->nunits + (long) b->host_div_units.size();
 	if (grid)
 		*grid = b->grid;
 	if (total_nnz)

@@ -496,13 +496,14 @@ __device__ __forceinline__ double div_acc_sum(const double *yw, int stride, int 
 	return s;
 }
 
+// Packet group g of a matrix, decoded by the calling wavefront with its
+// packets staged in the LDS at smem (the standalone kernel below and the
+// batched launch's DIV kernel).
 template <int K, bool IL>
-__global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__restrict__ bytes,
-		const int64_t *__restrict__ pk_off, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
-		long ngroups, int L, int buf_bytes, int acc_stride, int acc_copies, int xcd_k, const double *__restrict__ x,
-		double *__restrict__ y)
+__device__ __forceinline__ void div_group_lds(const uint8_t *__restrict__ bytes, const int64_t *__restrict__ pk_off,
+		const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row, long g, int L, int buf_bytes,
+		int acc_stride, int acc_copies, const double *__restrict__ x, double *__restrict__ y, u32x4 *smem)
 {
-	extern __shared__ u32x4 smem[];
 	uint8_t *pk = (uint8_t *) smem;                   // buf_bytes (a multiple of 16, padding included)
 	double *yw = (double *) (pk + buf_bytes);          // acc_copies copies of the row accumulators
 	const int lane = lane_id();
@@ -510,9 +511,6 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 	// same row collide on one LDS address (and serialise) acc_copies times
 	// less often -- with one copy those collisions bound the decoder
 	double *ywl = yw + (lane & (acc_copies - 1)) * acc_stride;
-	const long g = xcd_logical_block(xcd_k); // groups dealt to XCDs in chunks of xcd_k (0: in order)
-	if (g >= ngroups)
-		return;
 	const int p_end = grp_pk[g + 1];
 	int p = grp_pk[g];
 	// a group of one packet (the common case) stores its rows once at the
@@ -634,6 +632,19 @@ __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__re
 		}
 		wave_fence(); // the packet buffer and accumulators are reused
 	}
+}
+
+template <int K, bool IL>
+__global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__restrict__ bytes,
+		const int64_t *__restrict__ pk_off, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
+		long ngroups, int L, int buf_bytes, int acc_stride, int acc_copies, int xcd_k, const double *__restrict__ x,
+		double *__restrict__ y)
+{
+	extern __shared__ u32x4 smem[];
+	const long g = xcd_logical_block(xcd_k); // groups dealt to XCDs in chunks of xcd_k (0: in order)
+	if (g >= ngroups)
+		return;
+	div_group_lds<K, IL>(bytes, pk_off, grp_pk, grp_row, g, L, buf_bytes, acc_stride, acc_copies, x, y, smem);
 }
 
 // ------------------------------------------------------------ x-slot layout
@@ -992,6 +1003,100 @@ __global__ void __launch_bounds__(kBlock) div_slot_export_kernel(const uint8_t *
 			cs += n * rf;
 		}
 	}
+}
+
+// ------------------------------------------------------------ batched launch
+//
+// The DIV-RF members of a batch: their packet groups form one list, drawn by
+// the resident wavefronts (one per workgroup) by ticket, unit g from counter
+// g % nc, each wave requesting its next ticket before it decodes the current
+// group (the schedule of batch.hip's kernel). A group is decoded by
+// div_group_lds with the member's own lanes, accumulator copies and stream
+// layout; the LDS holds the largest member's packet buffer and accumulators.
+
+__global__ void div_batch_reset(int *counter, int nc, int stride)
+{
+	if ((int) threadIdx.x < nc)
+		counter[threadIdx.x * stride] = 0;
+}
+
+template <int K>
+__global__ void __launch_bounds__(kWave) div_batch_kernel(const BatchDivMat *__restrict__ mats,
+		const BatchDivUnit *__restrict__ units, int nunits, int nc, int stride, int *__restrict__ counter,
+		int buf_bytes)
+{
+	extern __shared__ u32x4 smem[];
+	const int c = (int) (blockIdx.x & (unsigned) (nc - 1));
+	int *ctr = counter + c * stride;
+	const BatchDivMat *cm = const_ptr<BatchDivMat>(ptr_bits(mats));
+	const BatchDivUnit *cu = const_ptr<BatchDivUnit>(ptr_bits(units));
+	int tk = 0;
+	if (lane_id() == 0)
+		tk = atomicAdd(ctr, 1);
+	int t = __builtin_amdgcn_readfirstlane(__shfl(tk, 0, kWave));
+	for (;;) {
+		const long g = (long) t * nc + c;
+		if (g >= nunits)
+			break; // wave-uniform: tickets only grow, every wave reaches it
+		int tn = 0;
+		if (lane_id() == 0)
+			tn = atomicAdd(ctr, 1); // the next ticket, in flight during this group
+		const BatchDivUnit u = cu[g];
+		const BatchDivMat M = cm[u.mat];
+		if (M.il)
+			div_group_lds<K, true>(global_ptr<const uint8_t>(M.bytes), global_ptr<const int64_t>(M.pk_off),
+					global_ptr<const int32_t>(M.grp_pk), global_ptr<const int32_t>(M.grp_row), u.group, M.L,
+					buf_bytes, M.acc_stride, M.acc_copies, global_ptr<const double>(M.x), global_ptr<double>(M.y),
+					smem);
+		else
+			div_group_lds<K, false>(global_ptr<const uint8_t>(M.bytes), global_ptr<const int64_t>(M.pk_off),
+					global_ptr<const int32_t>(M.grp_pk), global_ptr<const int32_t>(M.grp_row), u.group, M.L,
+					buf_bytes, M.acc_stride, M.acc_copies, global_ptr<const double>(M.x), global_ptr<double>(M.y),
+					smem);
+		t = __builtin_amdgcn_readfirstlane(__shfl(tn, 0, kWave));
+	}
+}
+
+int div_batch_member(const spmvhip_format *f, const double *d_x, double *d_y, BatchDivMat *out, int *buf_bytes,
+		int *acc_bytes)
+{
+	if (f->div_slots || f->div_buf_bytes <= 0)
+		return set_error(SPMVHIP_EINVAL, "batch: div_rf member needs the LDS-staged decoder without x slots "
+				"(div_layout 1 or 3, packets that fit the LDS)");
+	BatchDivMat M{};
+	M.bytes = ptr_bits(f->div_stream.p);
+	M.pk_off = ptr_bits(f->div_pk_off.p);
+	M.grp_pk = ptr_bits(f->div_groups.p);
+	M.grp_row = ptr_bits(f->div_group_rows.p);
+	M.x = ptr_bits(d_x);
+	M.y = ptr_bits(d_y);
+	M.L = f->div_lanes;
+	M.acc_stride = f->div_acc_stride;
+	M.acc_copies = f->p.div_acc_copies;
+	M.il = f->div_il ? 1 : 0;
+	*out = M;
+	*buf_bytes = f->div_buf_bytes;
+	*acc_bytes = f->div_acc_bytes;
+	return SPMVHIP_OK;
+}
+
+int div_batch_occupancy(int lds_bytes, int *per_cu)
+{
+	SPMVHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, div_batch_kernel<8>, kWave, lds_bytes));
+	return SPMVHIP_OK;
+}
+
+int div_batch_launch(const BatchDivMat *d_mats, const BatchDivUnit *d_units, int nunits, int nc, int stride,
+		int *d_counter, int grid, int buf_bytes, int acc_bytes, hipStream_t s)
+{
+	div_batch_reset<<<1, 1024, 0, s>>>(d_counter, nc, stride);
+	SPMVHIP_CHECK(hipGetLastError());
+	if (nunits > 0) {
+		div_batch_kernel<8><<<grid, kWave, (size_t) buf_bytes + (size_t) acc_bytes, s>>>(d_mats, d_units, nunits, nc,
+				stride, d_counter, buf_bytes);
+		SPMVHIP_CHECK(hipGetLastError());
+	}
+	return SPMVHIP_OK;
 }
 
 __global__ void fill_zero(double *y, long m)

@@ -80,4 +80,25 @@ int div_convert(spmvhip_format *f, const int32_t *h_row_ptr, const int32_t *h_co
 int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s);
 int div_export_decoded(const spmvhip_format *f, int which, void *buf, size_t *bytes);
 
+// DIV-RF members of a batched launch (batch.hip): one persistent kernel over
+// the packet groups of all of them (div.hip).
+struct BatchDivMat {
+	uint64_t bytes, pk_off, grp_pk, grp_row, x, y; // device arrays, as pointer values
+	int L, acc_stride, acc_copies, il;
+};
+struct BatchDivUnit {
+	int mat, group;
+};
+// The member record of a converted DIV_RF handle (its staged decoder's LDS
+// needs in *buf_bytes, *acc_bytes); SPMVHIP_EINVAL for a handle the batched
+// kernel cannot run (packets decoded from global memory, x-slot layout).
+int div_batch_member(const spmvhip_format *f, const double *d_x, double *d_y, BatchDivMat *out, int *buf_bytes,
+		int *acc_bytes);
+// Resident workgroups per CU of the batched DIV kernel with that much LDS.
+int div_batch_occupancy(int lds_bytes, int *per_cu);
+// Enqueues the counter reset (nc counters, `stride` ints apart) and the
+// batched DIV kernel.
+int div_batch_launch(const BatchDivMat *d_mats, const BatchDivUnit *d_units, int nunits, int nc, int stride,
+		int *d_counter, int grid, int buf_bytes, int acc_bytes, hipStream_t s);
+
 } // namespace spmvhip

@@ -18,10 +18,13 @@ import numpy as np
 from . import gen, hip, tuning
 
 
-def build(count, seed, warmup=2, iters=8, log=None, ranges=None, budget_s=None, start=0):
+def build(count, seed, warmup=2, iters=8, log=None, ranges=None, budget_s=None, start=0, stop=None):
     """Time `count` matrices drawn from sample_params(seed), beginning at draw
-    `start` (so a corpus can be built in several sessions and concatenated)."""
-    params = tuning.sample_params(seed, start + count * 2, **(ranges or {}))
+    `start`, or (stop given) every matrix of the draws [start, stop), so that
+    a corpus can be built in several sessions and concatenated."""
+    params = tuning.sample_params(seed, stop if stop is not None else start + count * 2, **(ranges or {}))
+    if stop is not None:
+        count = stop - start
     feats, times, lines, nnzs, pidx = [], [], [], [], []
     t0 = time.time()
     k = 0
@@ -59,6 +62,8 @@ def main():
     ap.add_argument("--iters", type=int, default=8)
     ap.add_argument("--budget", type=float, default=None, help="stop after this many seconds")
     ap.add_argument("--start", type=int, default=0, help="first parameter draw to use")
+    ap.add_argument("--stop", type=int, default=None,
+                    help="time every matrix of the draws [start, stop) (instead of --count matrices)")
     args = ap.parse_args()
     hip.init(0)
 
@@ -66,7 +71,7 @@ def main():
         print(s, file=sys.stderr, flush=True)
 
     X, T, L, N, I = build(args.count, args.seed, iters=args.iters, log=log, budget_s=args.budget,
-                          start=args.start)
+                          start=args.start, stop=args.stop)
     np.savez_compressed(args.out, features=X, times=T, lines=L, nnz=N, draw=I,
                         variants=np.array([tuning.variant_name(i) for i in range(len(tuning.VARIANTS))]))
     log("wrote %s: %d matrices x %d variants" % (args.out, T.shape[0], T.shape[1]))

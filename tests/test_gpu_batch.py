@@ -1,7 +1,8 @@
 """Batched SpMV: one persistent, ticket-scheduled launch over many
 generated matrices of uneven sizes, directly and replayed from a hipGraph.
 Every member's y is checked against the oracle (normwise <= 1e-13) and, for
-SELL members, bit-identical to the member's own standalone launch."""
+SELL members, bit-identical to the member's own standalone launch. DIV-RF
+members run in the sequence's second persistent kernel."""
 import json
 import os
 
@@ -15,7 +16,9 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 GEN = json.load(open(os.path.join(HERE, "golden", "generator.json")))
 VARIANTS = [("csr_vec", {}), ("sell_c_s", {"sigma": 256}), ("sell", {}), ("sell_c_s", {"sigma": 4096}),
-            ("csr_vec", {"block_nnz": 128}), ("csr_vec", {"block_nnz": 256, "lanes_per_row": 4})]
+            ("csr_vec", {"block_nnz": 128}), ("csr_vec", {"block_nnz": 256, "lanes_per_row": 4}),
+            ("div_rf", {"div_packet_vals": 1024, "unroll": 8}),
+            ("div_rf", {"div_packet_vals": 512, "div_layout": 1, "div_lanes": 32, "unroll": 4})]
 
 
 @pytest.fixture(scope="module")
@@ -42,7 +45,7 @@ def check(members):
     for mb in members:
         y = mb["dy"].download()
         assert pyoracle.normwise_rel_err(y, mb["y_ref"]) <= 1e-13, mb["name"]
-        if mb["kind"] != "csr_vec":
+        if mb["kind"] in ("sell", "sell_c_s"):
             np.testing.assert_array_equal(y, mb["y_alone"])
 
 
@@ -182,3 +185,32 @@ def test_batch_rejects_unbatchable(members):
         dy = hip.DeviceArray(a.m)
         with pytest.raises(hip.SpmvHipError):
             b.add(f, dx, dy)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_div_members_only(members, use_graph):
+    """A batch of DIV-RF members only (the CSR / SELL kernel has no units)."""
+    sel = [mb for mb in members if mb["kind"] == "div_rf"]
+    assert len(sel) >= 2
+    b = hip.Batch()
+    for mb in sel:
+        b.add(mb["fmt"], mb["dx"], mb["dy"])
+    b.finalize(use_graph=use_graph)
+    for rep in range(3):
+        for mb in sel:
+            mb["dy"].upload(np.full(mb["m"], np.nan))
+        b.run()
+        b.synchronize()
+        check(sel)
+    b.destroy()
+
+
+def test_batch_rejects_slot_layout_div():
+    a = gen.generate(gen.GenParams.parse(GEN["test_txt"]["line"]), stats=False)
+    f = hip.csr_to_format(a.row_ptr, a.col_ind, a.values, a.m, a.n, a.nnz, kind="div_rf", div_layout=2)
+    dx = hip.DeviceArray.from_host(gen.make_x(a.n, 5))
+    dy = hip.DeviceArray(a.m)
+    b = hip.Batch()
+    with pytest.raises(hip.SpmvHipError):
+        b.add(f, dx, dy)
+    b.destroy()
