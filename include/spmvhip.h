@@ -248,6 +248,17 @@ int spmvhip_bandwidth_probe(int mode, const double *src, double *dst, long n, in
 int spmvhip_csr_stats_device(const int32_t *d_row_ptr, const int32_t *d_col_ind, long m, long n,
 		double *out6);
 
+/* The same statistics with no allocation, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation). No counterpart in the reference. Enqueued on
+ * `stream` (a hipStream_t, NULL = the default stream) and asynchronous: the
+ * six doubles land in DEVICE memory d_out6. The per-workgroup partials live
+ * in a file-scope device array of 768 KiB that every GPU the library runs on
+ * holds until the process ends; it limits m to 67,108,864 rows (SPMVHIP_EINVAL
+ * beyond). Calls that overlap on one GPU share that array and race. */
+int spmvhip_csr_stats_device_noalloc(const int32_t *d_row_ptr, const int32_t *d_col_ind, long m, long n,
+		double *d_out6, void *stream);
+
 /* FP32 MLP n_in -> h1 -> h2 -> n_out (ReLU hidden, linear out) with
  * hand-written forward, backward (masked mean squared error) and Adam
  * kernels. Parameters: one flat array, per layer W[n_in][n_out] then b. */

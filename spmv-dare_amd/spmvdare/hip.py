@@ -83,6 +83,7 @@ _sig = {
                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
     "spmvhip_batch_destroy": ([_vp], None),
     "spmvhip_csr_stats_device": ([_vp, _vp, ctypes.c_long, ctypes.c_long, _vp], ctypes.c_int),
+    "spmvhip_csr_stats_device_noalloc": ([_vp, _vp, ctypes.c_long, ctypes.c_long, _vp, _vp], ctypes.c_int),
     "spmvhip_bandwidth_probe": ([ctypes.c_int, _vp, _vp, ctypes.c_long, ctypes.c_int, _vp], ctypes.c_int),
     "spmvhip_mlp_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint,
                             ctypes.POINTER(_vp)], ctypes.c_int),
@@ -356,6 +357,13 @@ def csr_stats_device(d_row_ptr, d_col_ind, m, n):
     keys = ("avg_nnz_per_row", "std_nnz_per_row", "max_nnz_per_row", "avg_bw_scaled",
             "avg_num_neighbours", "nonempty_rows")
     return dict(zip(keys, out.tolist()))
+
+
+def csr_stats_device_noalloc(d_row_ptr, d_col_ind, m, n, d_out6, stream=None):
+    """The allocation-free variant: the six statistics into the device
+    array d_out6 (6 doubles), asynchronously (synchronize before reading)."""
+    _check(_lib.spmvhip_csr_stats_device_noalloc(d_row_ptr.ptr, d_col_ind.ptr, m, n, d_out6.ptr, stream),
+           "csr_stats_noalloc")
 
 
 class MLP:
