@@ -32,6 +32,7 @@ enum {
 	SPMVGEN_ENOMEM = -2,     /* allocation failed */
 	SPMVGEN_EOVERFLOW = -3,  /* nnz does not fit the int32 row pointer */
 	SPMVGEN_EDUP = -4,       /* cross-row copy hit an occupied column (reference: printf("error"); exit(1)) */
+	SPMVGEN_EFILE = -5,      /* Matrix Market file missing or malformed (reference: error() and exit) */
 };
 
 /* Descriptor statistics of a generated matrix
@@ -89,6 +90,20 @@ int spmvgen_fill_x(double *x, long n, unsigned int seed);
 /* Descriptor statistics of any CSR matrix (same definitions as above). */
 int spmvgen_compute_stats(const int32_t *row_ptr, const int32_t *col_ind,
 		long m, long n, spmvgen_stats *out);
+
+/* Matrix Market import, the reference's real-matrix input path (import_file,
+ * src/bench.cpp:126-238: mtx_read, mtx_values_convert_to_real and coo_to_csr
+ * with sorted columns; lib/storage_formats/matrix_market/matrix_market.c:
+ * 152-331,421-456). Coordinate files with real, integer, complex (read as the
+ * modulus) or pattern (read as 1.0) fields and general, symmetric,
+ * skew-symmetric or Hermitian symmetry; a file without a banner is read as
+ * coordinate real general. With expand_symmetry each stored off-diagonal
+ * entry of a symmetric matrix also gives its mirror (negated for
+ * skew-symmetric). Rows ascending, columns ascending within a row,
+ * duplicates kept in file order. out->stats is zero; *symmetric (may be
+ * NULL) receives whether the file declares a symmetry. The arrays are
+ * released by spmvgen_free(). */
+int spmvgen_read_mtx(const char *path, int expand_symmetry, spmvgen_csr *out, int *symmetric);
 
 const char *spmvgen_last_error(void);
 

@@ -307,6 +307,15 @@ extern "C" const char *spmvgen_last_error(void)
 	return g_err;
 }
 
+namespace spmvgen_detail {
+// the last-error message of the library's other entry points (mtx.cpp)
+int set_error(int code, const char *msg)
+{
+	snprintf(g_err, sizeof(g_err), "%s", msg);
+	return code;
+}
+} // namespace spmvgen_detail
+
 extern "C" int spmvgen_degrees(long nr_rows, long nr_cols, double avg, double sd,
 		const char *distribution, unsigned int seed, double skew, int32_t *deg, long *nnz_out)
 {

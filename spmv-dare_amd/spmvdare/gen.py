@@ -8,6 +8,7 @@ src/bench.cpp:562-585):
     bw skew avg_num_neighbours cross_row_similarity seed
 """
 import ctypes
+import os
 from dataclasses import dataclass, fields
 
 import numpy as np
@@ -51,6 +52,8 @@ _lib.spmvgen_degrees.restype = ctypes.c_int
 _lib.spmvgen_compute_stats.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
                                        ctypes.c_long, ctypes.POINTER(_Stats)]
 _lib.spmvgen_compute_stats.restype = ctypes.c_int
+_lib.spmvgen_read_mtx.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_CSR), ctypes.POINTER(ctypes.c_int)]
+_lib.spmvgen_read_mtx.restype = ctypes.c_int
 
 
 class GenError(RuntimeError):
@@ -162,3 +165,25 @@ def structure_stats(row_ptr, col_ind, n):
     if rc != 0:
         raise GenError(_lib.spmvgen_last_error().decode())
     return {f[0]: getattr(s, f[0]) for f in _Stats._fields_}
+
+
+class MtxError(GenError):
+    pass
+
+
+def read_mtx(path, expand_symmetry=True):
+    """A Matrix Market file as CSR (spmvgen_read_mtx: the reference's
+    import_file semantics). Returns (CSRMatrix, symmetric flag of the file)."""
+    c = _CSR()
+    sym = ctypes.c_int()
+    rc = _lib.spmvgen_read_mtx(os.fsencode(path), 1 if expand_symmetry else 0, ctypes.byref(c), ctypes.byref(sym))
+    if rc != 0:
+        raise MtxError("spmvgen_read_mtx(%s): %s" % (path, _lib.spmvgen_last_error().decode()))
+    try:
+        m, nnz = c.m, c.nnz
+        rp = np.ctypeslib.as_array(c.row_ptr, shape=(m + 1,)).copy()
+        ci = np.ctypeslib.as_array(c.col_ind, shape=(max(nnz, 1),))[:nnz].copy()
+        va = np.ctypeslib.as_array(c.values, shape=(max(nnz, 1),))[:nnz].copy()
+    finally:
+        _lib.spmvgen_free(ctypes.byref(c))
+    return CSRMatrix(m, c.n, rp, ci, va, {}), bool(sym.value)
