@@ -58,7 +58,9 @@ enum spmvhip_kind {
 	SPMVHIP_SELL = 2,       /* src/spmv_kernels/sell.cpp: slices of C rows, no sort */
 	SPMVHIP_SELL_C_S = 3,   /* src/spmv_kernels/sell_c_s.cpp: SELL-C-sigma */
 	SPMVHIP_DIV_RF = 4,     /* src/spmv_kernels/DIV/div.cpp: DIV-RF packet stream */
-	SPMVHIP_NUM_KINDS = 5
+	SPMVHIP_SELL_SORTED = 5, /* src/spmv_kernels/sell_sorted.cpp: rows stable-sorted by ascending
+	                            length inside row windows, then C-row slices */
+	SPMVHIP_NUM_KINDS = 6
 };
 
 /* Tuning / format parameters. Zero-initialise and set `kind`; zero fields
@@ -66,7 +68,9 @@ enum spmvhip_kind {
 typedef struct spmvhip_params {
 	int kind;             /* enum spmvhip_kind */
 	int C;                /* SELL slice height; must be 64 (one wavefront) -- default 64 */
-	long sigma;           /* SELL-C-sigma sort window, 1..16384 -- default 4096 */
+	long sigma;           /* SELL_C_S: sort window, 1..16384 -- default 4096. SELL_SORTED: the
+	                         row window, a multiple of 64 up to 16384 -- default 4096 (the
+	                         reference sorts inside each OpenMP thread's row range) */
 	int lanes_per_row;    /* CSR_VEC: 0 = auto, else lanes of the LDS reduction per row (1..64);
 	                         SELL*: lanes per row T (power of two, 0/1 = the reference layout) */
 	int block_nnz;        /* CSR_VEC: nnz per row block staged in LDS (64..1024, default 256) */
@@ -103,7 +107,15 @@ typedef struct spmvhip_params {
 	                         reads fall in distinct LDS banks); 4 = both. 0 (default) = automatic:
 	                         the fastest measured layout whose packet fits the LDS, else 1.
 	                         Explicit 2..4 are an error when a packet does not fit */
-	int reserved[3];
+	int div_reorder;      /* DIV_RF: 1 = run the packet groups of each partition in ascending order
+	                         of their median column (the reference's DIV_PACKET_REORDERING=1,
+	                         src/spmv_kernels/DIV/div.cpp:377-395,455-498); y is unchanged.
+	                         0 (default) = in stream order */
+	double div_tolerance; /* DIV_RF: lossy value compression (the reference's DIV_VC_TOLERANCE,
+	                         div_kernels_rf.h:467-510): every value rounded to keep
+	                         ceil(|log2 tolerance|) fraction bits, relative error <= tolerance
+	                         (< 1); 0 (default) = lossless */
+	int reserved[2];
 } spmvhip_params;
 
 typedef struct spmvhip_format spmvhip_format;   /* opaque */
@@ -176,7 +188,7 @@ int spmvhip_time_spmv(spmvhip_format *f, const double *d_x, double *d_y,
 		int warmup, int iters, float *ms, float *total_ms);
 
 /* Converted arrays, copied to host for parity checks.
- * which: 0 row_order (int32, m)      [SELL_C_S]
+ * which: 0 row_order (int32, m)      [SELL_C_S, SELL_SORTED]
  *        1 slice_widths (int32, nslices)   [SELL, SELL_C_S]
  *        2 slice_ptr (int64, nslices+1)    [SELL, SELL_C_S]
  *        3 cols (int32, padded_nnz)        [SELL, SELL_C_S]
@@ -287,6 +299,24 @@ void spmvhip_mlp_destroy(spmvhip_mlp *net);
 int spmvhip_div_encode(const int32_t *row_ptr, const int32_t *col_ind, const double *values,
 		long m, long n, long nnz, int T, int L, long P, uint8_t *bytes, size_t *nbytes,
 		int64_t *pk_off, long *npackets, int32_t *grp_pk, long *ngroups);
+
+/* The packet-group order of the reference's packet reordering
+ * (DIV_PACKET_REORDERING=1, src/spmv_kernels/DIV/div.cpp:377-395,455-498)
+ * for the stream spmvhip_div_encode writes: order[k] = the k-th group to
+ * run; inside each partition, ascending median column of the partition's
+ * k-th packet, ties in stream order. With order == NULL only *ngroups is
+ * returned. Needs no device. */
+int spmvhip_div_group_order(const int32_t *row_ptr, const int32_t *col_ind, const double *values,
+		long m, long n, long nnz, int T, int L, long P, int32_t *order, long *ngroups);
+
+/* The value rounding of DIV_RF's lossy compression (DIV_VC_TOLERANCE,
+ * src/spmv_kernels/DIV/div_kernels_rf.h:467-510, applied to each value at
+ * :612-616): out[k] = in[k] with ceil(|log2 tolerance|) fraction bits kept,
+ * truncated or rounded up in its bit pattern, whichever leaves more trailing
+ * zero bits (rounded up on a tie). tolerance <= 0 copies; tolerance >= 1 or a
+ * rounding beyond it is SPMVHIP_EINVAL (the reference's error()). in == out
+ * is allowed. Needs no device. */
+int spmvhip_div_reduce_precision(const double *in, long n, double tolerance, double *out);
 
 /* The device form of the same stream in device layout `layout` (the
  * div_layout codes 1..4 of spmvhip_params: 1 the reference bytes, 2 x slots,

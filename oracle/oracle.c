@@ -310,6 +310,164 @@ void oracle_sell_spmv(long m, long VL, const int64_t *cluster_ptr, const int64_t
 	}
 }
 
+/* ------------------------------------------------------------ SELL_SORTED */
+
+/* Row windows of the reference's threads (sell_sorted.cpp:160-176): T
+ * nnz-balanced ranges of VL-row clusters, balanced on the cluster prefix
+ * sums row_ptr[c * VL]; the last range ends at row m. */
+void oracle_sellsorted_windows(const int32_t *rp, long m, long VL, long T, int64_t *row_start)
+{
+	long nc = (m + VL - 1) / VL;
+	int64_t *sums = malloc((size_t) (nc > 0 ? nc : 1) * sizeof(int64_t));
+	for (long c = 0; c < nc; c++)
+		sums[c] = rp[c * VL];
+	for (long t = 0; t < T; t++) {
+		long c0 = 0, c1 = 0;
+		if (nc > 0)
+			oracle_partition_prefix_sums(T, t, sums, nc, rp[m], &c0, &c1);
+		row_start[t] = c0 * VL < m ? c0 * VL : m;
+	}
+	row_start[T] = m;
+	free(sums);
+}
+
+/* Stable ascending sort of the rows of every window [row_start[w],
+ * row_start[w + 1]) by length (the degree-bucket sort of sell_sorted.cpp:
+ * 177-193): perm[i] = sorted position of row i. */
+void oracle_sellsorted_order(const int32_t *rp, long m, long nwin, const int64_t *row_start, int32_t *perm)
+{
+	(void) m;
+	for (long w = 0; w < nwin; w++) {
+		long i0 = row_start[w], i1 = row_start[w + 1];
+		int32_t dmax = 0;
+		for (long i = i0; i < i1; i++)
+			dmax = rp[i + 1] - rp[i] > dmax ? rp[i + 1] - rp[i] : dmax;
+		long *cnt = calloc((size_t) dmax + 2, sizeof(long));
+		for (long i = i0; i < i1; i++)
+			cnt[rp[i + 1] - rp[i] + 1]++;
+		for (long d = 0; d <= dmax; d++)
+			cnt[d + 1] += cnt[d];
+		for (long i = i0; i < i1; i++)
+			perm[i] = (int32_t) (i0 + cnt[rp[i + 1] - rp[i]]++);
+		free(cnt);
+	}
+}
+
+/* Cluster sizes VL * (longest sorted row of the cluster) and their prefix
+ * sums (sell_sorted.cpp:227-251). Returns nnz_ext. */
+long oracle_sellsorted_layout(const int32_t *rp, long m, long VL, const int32_t *perm, int64_t *cluster_ptr)
+{
+	long nc = (m + VL - 1) / VL;
+	int64_t *w = calloc((size_t) (nc > 0 ? nc : 1), sizeof(int64_t));
+	for (long i = 0; i < m; i++) {
+		int64_t d = rp[i + 1] - rp[i];
+		long c = perm[i] / VL;
+		w[c] = d > w[c] ? d : w[c];
+	}
+	int64_t total = 0;
+	for (long c = 0; c < nc; c++) {
+		cluster_ptr[c] = total;
+		total += VL * w[c];
+	}
+	cluster_ptr[nc] = total;
+	free(w);
+	return total;
+}
+
+/* Column-major fill of each cluster (sell_sorted.cpp:252-294): the sorted
+ * rows' entries, then padding (value 0, column = the last column written in
+ * this window so far, 0 at the window's start), missing rows of the last
+ * cluster likewise. */
+void oracle_sellsorted_fill(const int32_t *rp, const int32_t *col, const double *val, long m, long VL,
+		long nwin, const int64_t *row_start, const int32_t *perm, const int64_t *cluster_ptr,
+		int64_t *ja, double *a)
+{
+	long nc = (m + VL - 1) / VL;
+	int32_t *rev = malloc((size_t) (m > 0 ? m : 1) * sizeof(int32_t));
+	for (long i = 0; i < m; i++)
+		rev[perm[i]] = (int32_t) i;
+	int64_t nnz_ext = cluster_ptr[nc];
+	int64_t *rj = malloc((size_t) (nnz_ext > 0 ? nnz_ext : 1) * sizeof(int64_t));
+	double *ra = malloc((size_t) (nnz_ext > 0 ? nnz_ext : 1) * sizeof(double));
+	for (long win = 0; win < nwin; win++) {
+		long c0 = row_start[win] / VL, c1 = (row_start[win + 1] + VL - 1) / VL;
+		if (row_start[win] >= row_start[win + 1])
+			continue;
+		int64_t colv = 0;
+		for (long c = c0; c < c1; c++) {
+			int64_t w = (cluster_ptr[c + 1] - cluster_ptr[c]) / VL;
+			long i0 = VL * c, i1 = i0 + VL < m ? i0 + VL : m;
+			int64_t jj = cluster_ptr[c];
+			for (long i = i0; i < i1; i++) {
+				long r = rev[i], j = rp[r], k = 0;
+				for (; j < rp[r + 1]; j++, jj++, k++) {
+					ra[jj] = val[j];
+					colv = col[j];
+					rj[jj] = colv;
+				}
+				for (; k < w; k++, jj++) {
+					ra[jj] = 0;
+					rj[jj] = colv;
+				}
+			}
+			for (; jj < cluster_ptr[c + 1]; jj++) {
+				ra[jj] = 0;
+				rj[jj] = colv;
+			}
+			for (int64_t k = 0; k < w; k++)
+				for (long l = 0; l < VL; l++) {
+					ja[cluster_ptr[c] + k * VL + l] = rj[cluster_ptr[c] + l * w + k];
+					a[cluster_ptr[c] + k * VL + l] = ra[cluster_ptr[c] + l * w + k];
+				}
+		}
+	}
+	free(rj);
+	free(ra);
+	free(rev);
+}
+
+/* compute_sell of sell_sorted.cpp:346-427: per cluster, lane-wise fma over
+ * the width; sorted row i's result goes to y[rev_permutation[i]]. */
+void oracle_sellsorted_spmv(long m, long VL, const int64_t *cluster_ptr, const int64_t *ja, const double *a,
+		const int32_t *perm, const double *x, double *y)
+{
+	for (long r = 0; r < m; r++) {
+		long i = perm[r], c = i / VL, l = i % VL;
+		double acc = 0;
+		for (int64_t jj = cluster_ptr[c] + l; jj < cluster_ptr[c + 1]; jj += VL)
+			acc = fma(a[jj], x[ja[jj]], acc);
+		y[r] = acc;
+	}
+}
+
+/* ------------------------------------------------------------ DIV lossy values */
+
+double oracle_div_reduce_precision(double v, double tolerance)
+{
+	union {
+		double d;
+		uint64_t u;
+	} a, lo, hi;
+	a.d = v;
+	if (tolerance >= 1.0)
+		return NAN;
+	if (tolerance == 0 || a.u == 0)
+		return v;
+	if (__builtin_ctzll(a.u) >= 53)
+		return v;
+	uint64_t bits = (uint64_t) ceil(fabs(log2(tolerance)));
+	if (bits >= 52)
+		return v;
+	uint64_t step = 1ULL << (52 - bits);
+	lo.u = a.u & ~(step - 1);
+	hi.u = (a.u + (step - 1)) & ~(step - 1);
+	if (fabs(a.d - lo.d) / fabs(a.d) > tolerance || fabs(a.d - hi.d) / fabs(a.d) > tolerance)
+		return NAN;
+	int tz_lo = lo.u ? __builtin_ctzll(lo.u) : 64;
+	int tz_hi = hi.u ? __builtin_ctzll(hi.u) : 64;
+	return tz_lo > tz_hi ? lo.d : hi.d;
+}
+
 /* ------------------------------------------------------------ fixtures */
 
 uint64_t oracle_fnv1a(const void *p, long nbytes)

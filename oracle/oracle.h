@@ -17,9 +17,17 @@
  *                         tests/golden/), brute-force stable sort on tiny m.
  *   oracle_sell_*         closed form of the padding rule on hand-built
  *                         matrices (empty rows, ragged last slice) at any T.
+ *   oracle_sellsorted_*   bit-exact against the reference's own build
+ *                         (oracle/_ref/sell_sorted_dump: thread windows,
+ *                         permutation, cluster sizes, ja, a and y; golden
+ *                         fixtures in tests/golden/), brute-force stable
+ *                         sort on tiny m.
  *   oracle_partition_*    brute force over all targets on tiny arrays.
  *   oracle_div_*          round trip (decode(encode(A)) == A), the reference's
- *                         own self-check (src/spmv_kernels/DIV/div.cpp:515-602).
+ *                         own self-check (src/spmv_kernels/DIV/div.cpp:515-602);
+ *                         oracle_div_reduce_precision: the values decoded from
+ *                         the reference's own lossy streams (golden fixtures),
+ *                         and the error bound / trailing-zero invariants.
  */
 #ifndef SPMV_ORACLE_H
 #define SPMV_ORACLE_H
@@ -89,6 +97,20 @@ void oracle_sell_fill(const int32_t *rp, const int32_t *col, const double *val, 
 void oracle_sell_spmv(long m, long VL, const int64_t *cluster_ptr, const int64_t *ja,
 		const double *a, const double *x, double *y);
 
+/* SELL_SORTED (src/spmv_kernels/sell_sorted.cpp:119-316, 346-427): rows
+ * stable-sorted by ascending length inside row windows, then VL-row
+ * clusters as SELL. The reference's windows are its threads' ranges
+ * (oracle_sellsorted_windows); any cluster-aligned windows may be given.
+ * perm[i] = sorted position of row i. */
+void oracle_sellsorted_windows(const int32_t *rp, long m, long VL, long T, int64_t *row_start);
+void oracle_sellsorted_order(const int32_t *rp, long m, long nwin, const int64_t *row_start, int32_t *perm);
+long oracle_sellsorted_layout(const int32_t *rp, long m, long VL, const int32_t *perm, int64_t *cluster_ptr);
+void oracle_sellsorted_fill(const int32_t *rp, const int32_t *col, const double *val, long m, long VL,
+		long nwin, const int64_t *row_start, const int32_t *perm, const int64_t *cluster_ptr,
+		int64_t *ja, double *a);
+void oracle_sellsorted_spmv(long m, long VL, const int64_t *cluster_ptr, const int64_t *ja, const double *a,
+		const int32_t *perm, const double *x, double *y);
+
 /* Decode a DIV-RF packet stream with L value lanes (the inverse of
  * compress_kernel_div, src/spmv_kernels/DIV/div_kernels_rf.h:538-919, as the
  * reference's decompress path reads it, :948-1162 and :1247-1267): writes one
@@ -96,6 +118,12 @@ void oracle_sell_spmv(long m, long VL, const int64_t *cluster_ptr, const int64_t
  * packet's coordinate order. Returns the number written, -1 on overflow. */
 long oracle_div_decode(const unsigned char *bytes, long nbytes, long L, int32_t *rows, int32_t *cols,
 		double *vals, long cap);
+
+/* DIV_RF's lossy value rounding, reduce_precision of
+ * src/spmv_kernels/DIV/div_kernels_rf.h:467-510 for one value: returns the
+ * rounded value, or NaN when the reference would stop with error()
+ * (tolerance >= 1 or a candidate beyond the tolerance). */
+double oracle_div_reduce_precision(double v, double tolerance);
 
 /* FNV-1a 64 of a byte range: the hash the generator fixtures record. */
 uint64_t oracle_fnv1a(const void *p, long nbytes);

@@ -27,10 +27,17 @@ _sig = {
     "oracle_sell_layout": [_vp, _l, _l, _vp],
     "oracle_sell_fill": [_vp, _vp, _vp, _l, _l, _l, _vp, _vp, _vp],
     "oracle_sell_spmv": [_l, _l, _vp, _vp, _vp, _vp, _vp],
+    "oracle_sellsorted_windows": [_vp, _l, _l, _l, _vp],
+    "oracle_sellsorted_order": [_vp, _l, _l, _vp, _vp],
+    "oracle_sellsorted_layout": [_vp, _l, _l, _vp, _vp],
+    "oracle_sellsorted_fill": [_vp, _vp, _vp, _l, _l, _l, _vp, _vp, _vp, _vp, _vp],
+    "oracle_sellsorted_spmv": [_l, _l, _vp, _vp, _vp, _vp, _vp, _vp],
+    "oracle_div_reduce_precision": [ctypes.c_double, ctypes.c_double],
     "oracle_fnv1a": [_vp, _l],
 }
 _res = {"oracle_csr_naive_omp": ctypes.c_int, "oracle_sellcs_layout": _l, "oracle_sellcs_fill": _l,
-        "oracle_sell_layout": _l, "oracle_fnv1a": ctypes.c_uint64}
+        "oracle_sell_layout": _l, "oracle_sellsorted_layout": _l, "oracle_fnv1a": ctypes.c_uint64,
+        "oracle_div_reduce_precision": ctypes.c_double}
 for _n, _a in _sig.items():
     getattr(_lib, _n).argtypes = _a
     getattr(_lib, _n).restype = _res.get(_n)
@@ -140,6 +147,48 @@ def sell_spmv(conv, VL, x, m):
     _lib.oracle_sell_spmv(m, VL, conv["cluster_ptr"].ctypes.data, conv["ja"].ctypes.data,
                           conv["a"].ctypes.data, px, y.ctypes.data)
     return y
+
+
+def sellsorted_windows(rp, VL, T):
+    """Row starts of the reference's T thread windows (T + 1 entries)."""
+    rp, p0 = _c(rp, np.int32)
+    out = np.empty(T + 1, dtype=np.int64)
+    _lib.oracle_sellsorted_windows(p0, rp.size - 1, VL, T, out.ctypes.data)
+    return out
+
+
+def sellsorted_convert(rp, ci, va, VL, windows):
+    """SELL_SORTED over the given row windows (row starts, last = m)."""
+    rp, p0 = _c(rp, np.int32)
+    ci, p1 = _c(ci, np.int32)
+    va, p2 = _c(va, np.float64)
+    win, pw = _c(windows, np.int64)
+    m = rp.size - 1
+    perm = np.empty(max(m, 1), dtype=np.int32)[:m]
+    _lib.oracle_sellsorted_order(p0, m, win.size - 1, pw, perm.ctypes.data)
+    nc = (m + VL - 1) // VL
+    cptr = np.empty(nc + 1, dtype=np.int64)
+    total = _lib.oracle_sellsorted_layout(p0, m, VL, perm.ctypes.data, cptr.ctypes.data)
+    ja = np.empty(max(total, 1), dtype=np.int64)[:total]
+    a = np.empty(max(total, 1), dtype=np.float64)[:total]
+    _lib.oracle_sellsorted_fill(p0, p1, p2, m, VL, win.size - 1, pw, perm.ctypes.data, cptr.ctypes.data,
+                                ja.ctypes.data, a.ctypes.data)
+    order = np.empty(m, dtype=np.int32)
+    order[perm] = np.arange(m, dtype=np.int32)
+    return dict(perm=perm, order=order, cluster_ptr=cptr, ja=ja, a=a)
+
+
+def sellsorted_spmv(conv, VL, x):
+    x, px = _c(x, np.float64)
+    m = conv["perm"].size
+    y = np.zeros(m, dtype=np.float64)
+    _lib.oracle_sellsorted_spmv(m, VL, conv["cluster_ptr"].ctypes.data, conv["ja"].ctypes.data,
+                                conv["a"].ctypes.data, conv["perm"].ctypes.data, px, y.ctypes.data)
+    return y
+
+
+def div_reduce_precision(values, tolerance):
+    return np.array([_lib.oracle_div_reduce_precision(float(v), float(tolerance)) for v in np.ravel(values)])
 
 
 def fnv1a(a):

@@ -4,7 +4,8 @@
 // its file-scope state is reachable), on a CSR matrix read from a gen_dump
 // binary, and writes the packet streams it produced.
 //
-// Usage: OMP_NUM_THREADS=T div_dump csr.bin P out.bin
+// Usage: OMP_NUM_THREADS=T div_dump csr.bin P out.bin [reorder [tolerance]]
+// (the values of DIV_PACKET_REORDERING and DIV_VC_TOLERANCE, default 0)
 // out.bin: int64 T; per partition t: int64 nbytes, bytes, int64 ngroups,
 //          int64 packet_group_pos[ngroups + 1]
 // The packet header's two padding bytes (offsets 22-23 of every packet) are
@@ -19,12 +20,12 @@
 int main(int argc, char **argv)
 {
 	if (argc < 4) {
-		fprintf(stderr, "usage: %s csr.bin P out.bin\n", argv[0]);
+		fprintf(stderr, "usage: %s csr.bin P out.bin [reorder [tolerance]]\n", argv[0]);
 		return 2;
 	}
 	setenv("CSRCV_NUM_PACKET_VALS", argv[2], 1);
-	setenv("DIV_VC_TOLERANCE", "0", 1);
-	setenv("DIV_PACKET_REORDERING", "0", 1);
+	setenv("DIV_VC_TOLERANCE", argc > 5 ? argv[5] : "0", 1);
+	setenv("DIV_PACKET_REORDERING", argc > 4 ? argv[4] : "0", 1);
 	setenv("DIV_STEALING_ENABLED", "0", 1);
 	setenv("USE_PROCESSES", "0", 1);
 	FILE *f = fopen(argv[1], "rb");

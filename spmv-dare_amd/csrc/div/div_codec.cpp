@@ -5,6 +5,7 @@
 #include "div_codec.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <omp.h>
 
@@ -275,6 +276,45 @@ int compress(const int32_t *rp, const int32_t *col, const double *val, long i0, 
 
 } // namespace
 
+int reduce_precision(const double *in, long n, double tolerance, double *out, const char **err)
+{
+	*err = nullptr;
+	if (tolerance >= 1.0) {
+		*err = "DIV_VC_TOLERANCE must be < 1";
+		return -1;
+	}
+	if (!(tolerance > 0)) {
+		if (out != in && n > 0)
+			memcpy(out, in, (size_t) n * sizeof(double));
+		return 0;
+	}
+	const long keep = (long) ceil(fabs(log2(tolerance)));
+	int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+	for (long k = 0; k < n; k++) {
+		uint64_t u;
+		memcpy(&u, &in[k], 8);
+		double r = in[k];
+		if (u != 0 && __builtin_ctzll(u) < 53 && keep < 52) {
+			const uint64_t low = (1ull << (52 - keep)) - 1;
+			const uint64_t dn = u & ~low, up = (u + low) & ~low;
+			double vd, vu;
+			memcpy(&vd, &dn, 8);
+			memcpy(&vu, &up, 8);
+			if (fabs(in[k] - vd) / fabs(in[k]) > tolerance || fabs(in[k] - vu) / fabs(in[k]) > tolerance)
+				bad = 1;
+			const int tz_dn = dn ? __builtin_ctzll(dn) : 64, tz_up = up ? __builtin_ctzll(up) : 64;
+			r = tz_dn > tz_up ? vd : vu;
+		}
+		out[k] = r;
+	}
+	if (bad) {
+		*err = "a rounded value exceeds DIV_VC_TOLERANCE";
+		return -1;
+	}
+	return 0;
+}
+
 int encode(const int32_t *rp, const int32_t *col, const double *val, long m, long n, long nnz, int T, int L,
 		long P, Stream *out, const char **err)
 {
@@ -288,6 +328,7 @@ int encode(const int32_t *rp, const int32_t *col, const double *val, long m, lon
 	out->pk_nnz.clear();
 	out->grp_pk.clear();
 	out->grp_row.clear();
+	out->part_grp.clear();
 
 	struct Plan {
 		long i, j, nv;
@@ -295,7 +336,10 @@ int encode(const int32_t *rp, const int32_t *col, const double *val, long m, lon
 	};
 	std::vector<Plan> plan;
 	std::vector<long> grp_first_row;
-	for (long t = 0; t < T && m > 0; t++) {
+	for (long t = 0; t < T; t++) {
+		out->part_grp.push_back((int32_t) grp_first_row.size());
+		if (m == 0)
+			continue;
 		long i_s, i_e;
 		partition(T, t, rp, m, nnz, &i_s, &i_e);
 		long j = rp[i_s], j_e = rp[i_e];
@@ -356,7 +400,48 @@ int encode(const int32_t *rp, const int32_t *col, const double *val, long m, lon
 	for (size_t g = 0; g < grp_first_row.size(); g++)
 		out->grp_row.push_back(g == 0 ? 0 : (int32_t) grp_first_row[g]);
 	out->grp_row.push_back((int32_t) m);
+	out->part_grp.push_back((int32_t) grp_first_row.size());
 	return 0;
+}
+
+std::vector<int32_t> group_order_by_median(const Stream &st, const int32_t *col)
+{
+	const long ng = st.ngroups();
+	std::vector<int32_t> order(ng);
+	std::vector<double> v;
+	for (size_t t = 0; t + 1 < st.part_grp.size(); t++) {
+		const int32_t g0 = st.part_grp[t], g1 = st.part_grp[t + 1];
+		if (g0 == g1)
+			continue;
+		const int32_t p0 = st.grp_pk[g0];
+		std::vector<long> key(g1 - g0);
+		for (int32_t k = 0; k < g1 - g0; k++) {
+			const int32_t p = p0 + k; // the partition's k-th packet
+			const int64_t j0 = st.pk_nnz[p], N = st.pk_nnz[p + 1] - j0;
+			// median_unbiased: virtual index (N - 1) q + q / 3 + 1 / 3 = N / 2 at q = 1/2
+			const double virt = (double) (N - 1) * 0.5 + (1.0 / 3 * 0.5 + 1.0 / 3);
+			const long idx = (long) virt;
+			const double frac = virt - (double) idx;
+			v.assign(col + j0, col + j0 + N);
+			double med;
+			if (idx >= N - 1) {
+				med = *std::max_element(v.begin(), v.end());
+			} else {
+				std::nth_element(v.begin(), v.begin() + idx, v.end());
+				const double lo = v[idx];
+				const double hi = *std::min_element(v.begin() + idx + 1, v.end());
+				med = (1 - frac) * lo + frac * hi;
+			}
+			key[k] = (long) med;
+		}
+		std::vector<int32_t> ids(g1 - g0);
+		for (int32_t k = 0; k < g1 - g0; k++)
+			ids[k] = k;
+		std::stable_sort(ids.begin(), ids.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+		for (int32_t k = 0; k < g1 - g0; k++)
+			order[g0 + k] = g0 + ids[k];
+	}
+	return order;
 }
 
 namespace {

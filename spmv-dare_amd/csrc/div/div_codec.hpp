@@ -33,6 +33,7 @@ struct Stream {
 	std::vector<int64_t> pk_nnz;     // first nnz index of each packet, + nnz
 	std::vector<int32_t> grp_pk;     // first packet of each packet group, + npackets
 	std::vector<int32_t> grp_row;    // first row owned by each group, + m
+	std::vector<int32_t> part_grp;   // first group of each partition, + ngroups
 	long npackets() const { return (long) pk_off.size() - 1; }
 	long ngroups() const { return (long) grp_pk.size() - 1; }
 };
@@ -41,6 +42,30 @@ struct Stream {
 // column span > 32 bits in a packet, which the reference rejects with error()).
 int encode(const int32_t *row_ptr, const int32_t *col, const double *val, long m, long n, long nnz,
 		int T, int L, long P, Stream *out, const char **err);
+
+// Lossy value compression (DIV_VC_TOLERANCE, src/spmv_kernels/DIV/
+// div_kernels_rf.h:467-510, applied to every value before packetisation,
+// :612-616): with b = ceil(|log2 tolerance|) fraction bits kept, the low
+// 52 - b fraction bits of the value are cleared (truncation) or the value is
+// rounded up to the next multiple of 2^(52 - b) in its bit pattern, whichever
+// result has more trailing zero bits (the rounded-up one on a tie). Zero, values
+// whose fraction is already zero and b >= 52 are kept as they are. Values
+// then repeat more often and their byte lengths shrink. tolerance <= 0 is
+// lossless. Returns 0, or -1 with *err set when tolerance >= 1 or a rounded
+// value's relative error exceeds the tolerance (the reference's error()).
+int reduce_precision(const double *in, long n, double tolerance, double *out, const char **err);
+
+// The order in which the packet groups are processed with the reference's
+// packet reordering (DIV_PACKET_REORDERING=1, src/spmv_kernels/DIV/div.cpp:
+// 377-395, 455-498): inside each partition, the groups in ascending order of
+// a key, the key of the partition's k-th group being the median column
+// (numpy's "median_unbiased" quantile, lib/array_metrics.c:940-1000,
+// truncated to an integer) of the partition's k-th packet -- the reference
+// records one key per packet and reads them by group index, which is the
+// same packet whenever every group is one packet. Ties keep the group order
+// (the reference's quicksort leaves them to its partitioning). Groups own
+// disjoint rows, so the order changes which groups run side by side, not y.
+std::vector<int32_t> group_order_by_median(const Stream &st, const int32_t *col);
 
 // Device layouts of a packet (the GPU decoders' forms of a reference packet;
 // values and value lengths are the reference's bytes in every form).

@@ -60,6 +60,7 @@ struct BatchMat {
 	uint64_t partials;  // CSR with split rows: segment partial sums, double (0 otherwise)
 	long m;
 	int lanes, split_nnz;
+	int narrow, pad_; // CSR: one stored entry per lane and load (the member's narrow_loads)
 	// read by batch_split_finalize only
 	uint64_t seg_ptr, split_rows; // int32
 };
@@ -134,10 +135,15 @@ __device__ __forceinline__ void batch_unit(const BatchMat *__restrict__ mats, co
 	const int32_t *col = global_ptr<const int32_t>(M.col);
 	const double *val = global_ptr<const double>(M.val);
 	if (kind == kUnitCsr) {
-		csr_block_at<BNNZ>(global_ptr<const int32_t>(M.rp), col, val, x, y, r.r0, r.r1, r.j0, r.j1, M.split_nnz,
-				M.partials != 0, M.lanes, prod);
+		if (M.narrow)
+			csr_block_at<BNNZ, true>(global_ptr<const int32_t>(M.rp), col, val, x, y, r.r0, r.r1, r.j0, r.j1,
+					M.split_nnz, M.partials != 0, M.lanes, prod);
+		else
+			csr_block_at<BNNZ, false>(global_ptr<const int32_t>(M.rp), col, val, x, y, r.r0, r.r1, r.j0, r.j1,
+					M.split_nnz, M.partials != 0, M.lanes, prod);
 	} else if (kind == kUnitSeg) {
-		const double sum = wave_row_sum_pairs(col, val, x, r.j0, r.j1);
+		const double sum = M.narrow ? wave_row_sum(col, val, x, r.j0, r.j1)
+		                            : wave_row_sum_pairs(col, val, x, r.j0, r.j1);
 		if (lane_id() == 0)
 			global_ptr<double>(M.partials)[r.u] = sum;
 	} else {
@@ -359,6 +365,7 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 		M.val = ptr_bits(f->val.p);
 		M.lanes = f->lanes;
 		M.split_nnz = f->split_nnz;
+		M.narrow = f->p.narrow_loads != 0;
 		if (f->nsegs > 0) {
 			M.seg_ptr = ptr_bits(f->seg_ptr.p);
 			M.split_rows = ptr_bits(f->split_rows.p);
@@ -366,13 +373,13 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 			nsegs = f->nsegs;
 		}
 		units = f->nblocks;
-	} else if (f->kind == SPMVHIP_SELL || f->kind == SPMVHIP_SELL_C_S) {
+	} else if (is_sell(f->kind)) {
 		kind = kUnitSell;
 		M.rp = ptr_bits(f->slice_ptr.p);
 		M.sw = ptr_bits(f->slice_w.p);
 		M.col = ptr_bits(f->s_cols.p);
 		M.val = ptr_bits(f->s_vals.p);
-		M.order = f->kind == SPMVHIP_SELL_C_S ? ptr_bits(f->row_order.p) : 0;
+		M.order = f->row_order.p ? ptr_bits(f->row_order.p) : 0;
 		M.lanes = f->lanes; // lanes per row T (>= 1) of the slice layout
 		units = f->nslices;
 	} else if (f->kind == SPMVHIP_DIV_RF) {
@@ -385,8 +392,8 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 			return set_error(SPMVHIP_EINVAL, "batch: too many DIV packet groups");
 		const int di = (int) b->host_div.size();
 		b->host_div.push_back(D);
-		for (long g = 0; g < f->div_ngroups; g++)
-			b->host_div_units.push_back({di, (int) g});
+		for (long k = 0; k < f->div_ngroups; k++) // in the member's group order (packet reordering)
+			b->host_div_units.push_back({di, f->div_order_host.empty() ? (int) k : f->div_order_host[k]});
 		b->div_buf = std::max(b->div_buf, buf);
 		b->div_acc = std::max(b->div_acc, acc);
 		b->fmts.push_back(const_cast<spmvhip_format *>(f));

@@ -29,6 +29,7 @@ static const char *kind_name(int kind)
 	case SPMVHIP_SELL: return "HIP_SELL";
 	case SPMVHIP_SELL_C_S: return "HIP_SELL_C_s";
 	case SPMVHIP_DIV_RF: return "HIP_DIV_RF";
+	case SPMVHIP_SELL_SORTED: return "HIP_SELL_SORTED";
 	}
 	return "?";
 }
@@ -69,8 +70,9 @@ static int convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col,
 	switch (f->kind) {
 	case SPMVHIP_CSR_NAIVE: rc = csr_naive_convert(f); break;
 	case SPMVHIP_CSR_VEC: rc = csr_vec_convert(f); break;
-	case SPMVHIP_SELL: rc = sell_convert(f, false); break;
-	case SPMVHIP_SELL_C_S: rc = sell_convert(f, true); break;
+	case SPMVHIP_SELL: rc = sell_convert(f, kSellUnsorted); break;
+	case SPMVHIP_SELL_C_S: rc = sell_convert(f, kSellDescending); break;
+	case SPMVHIP_SELL_SORTED: rc = sell_convert(f, kSellAscending); break;
 	case SPMVHIP_DIV_RF: rc = div_convert(f, h_rp, h_col, h_val); break;
 	default: rc = set_error(SPMVHIP_EINVAL, "unknown format kind %d", f->kind);
 	}
@@ -125,7 +127,8 @@ static int run_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s
 	case SPMVHIP_CSR_NAIVE: return csr_naive_spmv(f, x, y, s);
 	case SPMVHIP_CSR_VEC: return csr_vec_spmv(f, x, y, s);
 	case SPMVHIP_SELL:
-	case SPMVHIP_SELL_C_S: return sell_spmv(f, x, y, s);
+	case SPMVHIP_SELL_C_S:
+	case SPMVHIP_SELL_SORTED: return sell_spmv(f, x, y, s);
 	case SPMVHIP_DIV_RF: return div_spmv(f, x, y, s);
 	}
 	return set_error(SPMVHIP_EINVAL, "unknown format kind %d", f->kind);
@@ -438,7 +441,8 @@ int spmvhip_statistics_print_data(spmvhip_format *f, char *buf, long buf_n)
 	switch (f->kind) {
 	case SPMVHIP_CSR_VEC: snprintf(variant, sizeof(variant), "bnnz%d_L%d", f->block_nnz, f->lanes); break;
 	case SPMVHIP_SELL:
-	case SPMVHIP_SELL_C_S: snprintf(variant, sizeof(variant), "C%d_s%ld_u%d", f->p.C, f->p.sigma, f->p.unroll); break;
+	case SPMVHIP_SELL_C_S:
+	case SPMVHIP_SELL_SORTED: snprintf(variant, sizeof(variant), "C%d_s%ld_u%d", f->p.C, f->p.sigma, f->p.unroll); break;
 	case SPMVHIP_DIV_RF: snprintf(variant, sizeof(variant), "L%d_P%ld", f->div_lanes, f->p.div_packet_vals); break;
 	default: snprintf(variant, sizeof(variant), "naive");
 	}
@@ -459,7 +463,7 @@ int spmvhip_get_info(const spmvhip_format *f, spmvhip_info *info)
 	info->mem_footprint = f->mem_footprint;
 	info->csr_mem_footprint = f->csr_mem_footprint;
 	info->conv_ms = f->conv_ms;
-	info->padded_nnz = (f->kind == SPMVHIP_SELL || f->kind == SPMVHIP_SELL_C_S) ? f->padded : f->nnz;
+	info->padded_nnz = is_sell(f->kind) ? f->padded : f->nnz;
 	info->num_work_items = f->kind == SPMVHIP_CSR_VEC ? f->nblocks
 		: (f->kind == SPMVHIP_DIV_RF ? f->div_npackets : f->nslices);
 	info->num_split_rows = f->nsplit;
@@ -472,11 +476,11 @@ int spmvhip_export(const spmvhip_format *f, int which, void *buf, size_t *bytes_
 		return set_error(SPMVHIP_EINVAL, "NULL argument");
 	const void *src = nullptr;
 	size_t need = 0;
-	bool sell = f->kind == SPMVHIP_SELL || f->kind == SPMVHIP_SELL_C_S;
+	bool sell = is_sell(f->kind);
 	switch (which) {
 	case 0:
-		if (f->kind != SPMVHIP_SELL_C_S)
-			return set_error(SPMVHIP_EINVAL, "row_order exists only for SELL_C_S");
+		if (!f->row_order.p)
+			return set_error(SPMVHIP_EINVAL, "row_order exists only for SELL_C_S and SELL_SORTED");
 		src = f->row_order.p;
 		need = f->m * sizeof(int32_t);
 		break;

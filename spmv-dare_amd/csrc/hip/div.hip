@@ -161,13 +161,15 @@ __device__ __forceinline__ uint64_t load_le_fast(const uint8_t *__restrict__ bas
 template <int K>
 __global__ void __launch_bounds__(kBlock) div_spmv_kernel(const uint8_t *__restrict__ bytes,
 		const int64_t *__restrict__ pk_off, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
-		long ngroups, int L, const double *__restrict__ x, double *__restrict__ y)
+		long ngroups, const int32_t *__restrict__ grp_order, int L, const double *__restrict__ x,
+		double *__restrict__ y)
 {
 	__shared__ double yl[kDivWavesPerBlock][256];
 	const int lane = lane_id(), wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-	const long g = (long) blockIdx.x * kDivWavesPerBlock + wv;
-	if (g >= ngroups)
+	const long k = (long) blockIdx.x * kDivWavesPerBlock + wv;
+	if (k >= ngroups)
 		return; // waves do not synchronise with each other
+	const long g = grp_order ? grp_order[k] : k;
 	for (long r = grp_row[g] + lane; r < grp_row[g + 1]; r += kWave)
 		y[r] = 0.0;
 	wave_fence();
@@ -637,13 +639,14 @@ __device__ __forceinline__ void div_group_lds(const uint8_t *__restrict__ bytes,
 template <int K, bool IL>
 __global__ void __launch_bounds__(kWave) div_spmv_lds_kernel(const uint8_t *__restrict__ bytes,
 		const int64_t *__restrict__ pk_off, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
-		long ngroups, int L, int buf_bytes, int acc_stride, int acc_copies, int xcd_k, const double *__restrict__ x,
-		double *__restrict__ y)
+		long ngroups, const int32_t *__restrict__ grp_order, int L, int buf_bytes, int acc_stride, int acc_copies,
+		int xcd_k, const double *__restrict__ x, double *__restrict__ y)
 {
 	extern __shared__ u32x4 smem[];
-	const long g = xcd_logical_block(xcd_k); // groups dealt to XCDs in chunks of xcd_k (0: in order)
-	if (g >= ngroups)
+	const long k = xcd_logical_block(xcd_k); // groups dealt to XCDs in chunks of xcd_k (0: in order)
+	if (k >= ngroups)
 		return;
+	const long g = grp_order ? grp_order[k] : k;
 	div_group_lds<K, IL>(bytes, pk_off, grp_pk, grp_row, g, L, buf_bytes, acc_stride, acc_copies, x, y, smem);
 }
 
@@ -817,8 +820,8 @@ __device__ __forceinline__ void slot_rf1_group(const uint8_t *__restrict__ pk, c
 template <int K, bool IL>
 __global__ void __launch_bounds__(kWave) div_slot_kernel(const uint8_t *__restrict__ bytes,
 		const int64_t *__restrict__ pk_off, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,
-		long ngroups, int L, int buf_bytes, int xs_bytes, int acc_stride, int acc_copies, int xcd_k,
-		const double *__restrict__ x, double *__restrict__ y)
+		long ngroups, const int32_t *__restrict__ grp_order, int L, int buf_bytes, int xs_bytes, int acc_stride,
+		int acc_copies, int xcd_k, const double *__restrict__ x, double *__restrict__ y)
 {
 	extern __shared__ u32x4 smem[];
 	uint8_t *pk = (uint8_t *) smem;                    // buf_bytes (a multiple of 16, padding included)
@@ -826,9 +829,10 @@ __global__ void __launch_bounds__(kWave) div_slot_kernel(const uint8_t *__restri
 	double *yw = (double *) (pk + buf_bytes + xs_bytes); // acc_copies copies of the row accumulators
 	const int lane = lane_id();
 	double *ywl = yw + (lane & (acc_copies - 1)) * acc_stride;
-	const long g = xcd_logical_block(xcd_k);
-	if (g >= ngroups)
+	const long k = xcd_logical_block(xcd_k);
+	if (k >= ngroups)
 		return;
+	const long g = grp_order ? grp_order[k] : k;
 	const int p_end = grp_pk[g + 1];
 	int p = grp_pk[g];
 	const bool single = p_end - p == 1;
@@ -1127,9 +1131,22 @@ int div_convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col, co
 	f->div_lanes = L;
 	spmvdiv::Stream st;
 	const char *err = nullptr;
+	std::vector<double> lossy;
+	if (f->p.div_tolerance != 0) {
+		lossy.resize(f->nnz);
+		if (spmvdiv::reduce_precision(h_val, f->nnz, f->p.div_tolerance, lossy.data(), &err))
+			return set_error(SPMVHIP_EINVAL, "div_rf: %s (tolerance %g)", err, f->p.div_tolerance);
+		h_val = lossy.data();
+	}
 	if (spmvdiv::encode(h_rp, h_col, h_val, f->m, f->n, f->nnz, T, L, P, &st, &err))
 		return set_error(SPMVHIP_EFORMAT, "div_rf: %s (the reference rejects this matrix too)", err);
 	const long np = st.npackets(), ng = st.ngroups();
+	if (f->p.div_reorder < 0 || f->p.div_reorder > 1)
+		return set_error(SPMVHIP_EINVAL, "div_rf: div_reorder %d (0 stream order, 1 by median column)",
+				f->p.div_reorder);
+	f->div_order_host.clear();
+	if (f->p.div_reorder && ng > 0)
+		f->div_order_host = spmvdiv::group_order_by_median(st, h_col);
 	if (f->p.div_layout < 0 || f->p.div_layout > 4)
 		return set_error(SPMVHIP_EINVAL, "div_rf: div_layout %d (0 automatic, 1 reference bytes, 2 x slots, "
 				"3 interleaved lane streams, 4 x slots and interleaved lane streams)", f->p.div_layout);
@@ -1228,6 +1245,11 @@ int div_convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col, co
 	SPMVHIP_CHECK(hipMemcpyAsync(f->div_groups.p, st.grp_pk.data(), (ng + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
 	SPMVHIP_CHECK(hipMemcpyAsync(f->div_group_rows.p, st.grp_row.data(), (ng + 1) * sizeof(int32_t),
 			hipMemcpyHostToDevice, s));
+	if (!f->div_order_host.empty()) {
+		SPMVHIP_TRY(f->div_order.alloc(ng * sizeof(int32_t)));
+		SPMVHIP_CHECK(hipMemcpyAsync(f->div_order.p, f->div_order_host.data(), ng * sizeof(int32_t),
+				hipMemcpyHostToDevice, s));
+	}
 	SPMVHIP_CHECK(hipStreamSynchronize(s));
 	f->row_ptr.release();
 	f->col.release();
@@ -1245,13 +1267,14 @@ int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 		SPMVHIP_CHECK(hipGetLastError());
 		return SPMVHIP_OK;
 	}
+	const int32_t *order = f->div_order.p ? f->div_order.as<int32_t>() : nullptr;
 	if (f->div_slots) {
 		const size_t lds = (size_t) f->div_buf_bytes + (size_t) f->div_xs_bytes + (size_t) f->div_acc_bytes;
 		const int xk = xcd_chunk(f->p.xcd_remap);
 		const unsigned grid = xcd_padded_grid((unsigned) f->div_ngroups, xk);
 #define LAUNCH_SLOT(K, IL)                                                                                        \
 	div_slot_kernel<K, IL><<<grid, kWave, lds, s>>>(f->div_stream.as<uint8_t>(), f->div_pk_off.as<int64_t>(),  \
-			f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(), f->div_ngroups, f->div_lanes,     \
+			f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(), f->div_ngroups, order, f->div_lanes, \
 			f->div_buf_bytes, f->div_xs_bytes, f->div_acc_stride, f->p.div_acc_copies, xk, x, y)
 #define LAUNCH_SLOT_K(IL)          \
 	if (f->p.unroll >= 16)     \
@@ -1273,7 +1296,7 @@ int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 		const unsigned grid = xcd_padded_grid((unsigned) f->div_ngroups, xk);
 #define LAUNCH_LDS(K, IL)                                                                                       \
 	div_spmv_lds_kernel<K, IL><<<grid, kWave, lds, s>>>(f->div_stream.as<uint8_t>(), f->div_pk_off.as<int64_t>(), \
-			f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(), f->div_ngroups, f->div_lanes,      \
+			f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(), f->div_ngroups, order, f->div_lanes, \
 			f->div_buf_bytes, f->div_acc_stride, f->p.div_acc_copies, xk, x, y)
 #define LAUNCH_LDS_K(IL)           \
 	if (f->p.unroll >= 16)     \
@@ -1292,7 +1315,7 @@ int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 	} else {
 		div_spmv_kernel<8><<<grid_for(f->div_ngroups, kDivWavesPerBlock), kBlock, 0, s>>>(f->div_stream.as<uint8_t>(),
 				f->div_pk_off.as<int64_t>(), f->div_groups.as<int32_t>(), f->div_group_rows.as<int32_t>(),
-				f->div_ngroups, f->div_lanes, x, y);
+				f->div_ngroups, order, f->div_lanes, x, y);
 	}
 	SPMVHIP_CHECK(hipGetLastError());
 	return SPMVHIP_OK;
@@ -1361,6 +1384,40 @@ extern "C" int spmvhip_div_encode(const int32_t *row_ptr, const int32_t *col_ind
 	}
 	*nbytes = st.bytes.size();
 	*npackets = st.npackets();
+	*ngroups = st.ngroups();
+	return SPMVHIP_OK;
+}
+
+extern "C" int spmvhip_div_reduce_precision(const double *in, long n, double tolerance, double *out)
+{
+	using spmvhip::set_error;
+	if ((n > 0 && (!in || !out)) || n < 0)
+		return set_error(SPMVHIP_EINVAL, "NULL argument or negative length");
+	const char *err = nullptr;
+	if (spmvdiv::reduce_precision(in, n, tolerance, out, &err))
+		return set_error(SPMVHIP_EINVAL, "div_rf: %s (tolerance %g)", err, tolerance);
+	return SPMVHIP_OK;
+}
+
+extern "C" int spmvhip_div_group_order(const int32_t *row_ptr, const int32_t *col_ind, const double *values, long m,
+		long n, long nnz, int T, int L, long P, int32_t *order, long *ngroups)
+{
+	using spmvhip::set_error;
+	if (!row_ptr || (nnz > 0 && (!col_ind || !values)) || !ngroups)
+		return set_error(SPMVHIP_EINVAL, "NULL argument");
+	if (T < 1 || L < 1 || P < 1 || m < 0)
+		return set_error(SPMVHIP_EINVAL, "T, L, P must be positive");
+	spmvdiv::Stream st;
+	const char *err = nullptr;
+	if (spmvdiv::encode(row_ptr, col_ind, values, m, n, nnz, T, L, P, &st, &err))
+		return set_error(SPMVHIP_EFORMAT, "div_rf: %s", err);
+	if (order) {
+		if (*ngroups < st.ngroups())
+			return set_error(SPMVHIP_EINVAL, "output buffer too small");
+		const std::vector<int32_t> o = spmvdiv::group_order_by_median(st, col_ind);
+		if (!o.empty())
+			memcpy(order, o.data(), o.size() * sizeof(int32_t));
+	}
 	*ngroups = st.ngroups();
 	return SPMVHIP_OK;
 }

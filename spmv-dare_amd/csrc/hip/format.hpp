@@ -3,6 +3,8 @@
 
 #include "common.hpp"
 
+#include <vector>
+
 struct spmvhip_format {
 	spmvhip_params p{};
 	int kind = 0;
@@ -44,6 +46,8 @@ struct spmvhip_format {
 	spmvhip::DevBuf div_pk_nnz;   // int64 per packet: first nnz index
 	spmvhip::DevBuf div_groups;   // int32 per packet group: first packet
 	spmvhip::DevBuf div_group_rows; // int32 per packet group: [row_begin,row_end)
+	spmvhip::DevBuf div_order;    // int32 per packet group: the k-th group to run (packet reordering)
+	std::vector<int32_t> div_order_host;
 	long div_npackets = 0, div_ngroups = 0, div_bytes = 0;
 	int div_lanes = 64;
 	int div_buf_bytes = 0;        // LDS packet buffer of the staged decoder: largest packet + 16, or 0 if too large
@@ -73,7 +77,15 @@ int csr_naive_convert(spmvhip_format *f);
 int csr_naive_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s);
 int csr_vec_convert(spmvhip_format *f);
 int csr_vec_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s);
-int sell_convert(spmvhip_format *f, bool sorted);
+// Row order of the SELL family: none (SELL), stable descending by length in
+// sigma windows (SELL_C_S), stable ascending by length in row windows
+// (SELL_SORTED).
+enum SellOrder { kSellUnsorted = 0, kSellDescending = 1, kSellAscending = 2 };
+inline bool is_sell(int kind)
+{
+	return kind == SPMVHIP_SELL || kind == SPMVHIP_SELL_C_S || kind == SPMVHIP_SELL_SORTED;
+}
+int sell_convert(spmvhip_format *f, SellOrder order);
 int sell_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s);
 int div_convert(spmvhip_format *f, const int32_t *h_row_ptr, const int32_t *h_col,
 		const double *h_val);

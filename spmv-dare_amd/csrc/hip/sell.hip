@@ -1,4 +1,5 @@
-// SELL and SELL-C-sigma on gfx950, slice height C = 64 = one wavefront.
+// SELL, SELL-C-sigma and SELL_SORTED on gfx950, slice height C = 64 = one
+// wavefront.
 //
 // Layout in HBM (column-major inside a slice, as the reference's
 // csr_to_sellcs writes it, src/spmv_kernels/sell-C-s/sellcs_format.c:173-186):
@@ -22,10 +23,18 @@
 //   row, i.e. the row's last column, or for an empty row the last column of
 //   the nearest non-empty row above (0 if none) -- the reference with one
 //   OpenMP thread (its per-thread `col` restarts at 0 for each thread).
+//   SELL_SORTED (sell_sorted.cpp:119-316): inside each window of sigma rows
+//   (a multiple of 64; the reference's windows are its threads' nnz-balanced
+//   ranges of slices) the rows are stable-sorted by ascending length -- the
+//   same LDS radix sort with the digit order reversed; width = longest row
+//   of the slice (its last); padding value 0 with the column of the previous
+//   entry written in the window: the row's own last column, 0 for the empty
+//   rows that open a window, and for the missing rows of the last slice the
+//   last column of the last sorted row.
 //
 // spmv: one wave per slice, lane = row, one fused multiply-add per entry in
 // entry order (the reference's vfmacc, sellcs_mv_kernels_epi.c:292-297), then
-// y[row_order[row]] = acc for SELL-C-sigma and y[row] = acc for SELL. Only
+// y[row_order[row]] = acc for the sorted formats and y[row] = acc for SELL. Only
 // rows < m are written.
 
 #include "format.hpp"
@@ -48,10 +57,11 @@ __global__ void row_lengths(const int32_t *__restrict__ rp, long m, int32_t *__r
 		len[r] = rp[r + 1] - rp[r];
 }
 
-// Stable descending sort of one window of row lengths; writes the row ids.
+// Stable sort of one window of row lengths, descending (SELL-C-sigma) or
+// ascending (SELL_SORTED); writes the row ids. Ties keep the ascending row id.
 // LDS: two id arrays of `sigma` ints plus a 16 x 256 digit-count table.
 __global__ void __launch_bounds__(kSortThreads) sigma_window_sort(const int32_t *__restrict__ len, long m, int sigma,
-		int32_t *__restrict__ order)
+		int ascending, int32_t *__restrict__ order)
 {
 	extern __shared__ int32_t lds[];
 	int32_t *ids = lds;
@@ -92,10 +102,11 @@ __global__ void __launch_bounds__(kSortThreads) sigma_window_sort(const int32_t 
 			c[d] = 0;
 		for (int i = i0; i < i1; i++)
 			c[(len[ids[i]] >> shift) & 15]++;
-		// table in scan order: digit 15 first (descending), then thread id
+		// table in scan order: digit 15 first (descending) or digit 0 first
+		// (ascending), then thread id
 #pragma unroll
 		for (int d = 0; d < 16; d++)
-			cnt[(15 - d) * kSortThreads + t] = c[d];
+			cnt[(ascending ? d : 15 - d) * kSortThreads + t] = c[d];
 		__syncthreads();
 		// exclusive scan of the 16*256 table: thread t scans entries [16t, 16t+16)
 		int run = 0;
@@ -113,7 +124,7 @@ __global__ void __launch_bounds__(kSortThreads) sigma_window_sort(const int32_t 
 		__syncthreads();
 #pragma unroll
 		for (int d = 0; d < 16; d++)
-			c[d] = cnt[(15 - d) * kSortThreads + t];
+			c[d] = cnt[(ascending ? d : 15 - d) * kSortThreads + t];
 		for (int i = i0; i < i1; i++) {
 			int32_t id = ids[i];
 			int d = (len[id] >> shift) & 15;
@@ -160,7 +171,7 @@ __global__ void nonempty_keys(const int32_t *__restrict__ len, long m, int64_t *
 __global__ void __launch_bounds__(kBlock) sell_fill(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
 		const double *__restrict__ val, const int32_t *__restrict__ order, const int64_t *__restrict__ last_ne,
 		const int32_t *__restrict__ width, const int64_t *__restrict__ sptr, long m, long nslices, int T,
-		int32_t *__restrict__ scol, double *__restrict__ sval)
+		int ascending, int32_t *__restrict__ scol, double *__restrict__ sval)
 {
 	const long s = (long) blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
 	if (s >= nslices)
@@ -183,6 +194,14 @@ __global__ void __launch_bounds__(kBlock) sell_fill(const int32_t *__restrict__ 
 		long rr = r < m ? r : m - 1;
 		int64_t ne = m > 0 ? last_ne[rr] : -1;
 		pad_col = ne >= 0 ? col[rp[ne + 1] - 1] : 0;
+	} else if (ascending) {
+		if (len > 0) {
+			pad_col = col[j0 + len - 1];
+		} else if (r >= m) { // missing rows of the last slice: the last sorted row's last column
+			const long last = order[m - 1];
+			if (rp[last + 1] > rp[last])
+				pad_col = col[rp[last + 1] - 1];
+		}
 	} else if (T > 1 && len > 0) {
 		pad_col = col[j0 + len - 1]; // keeps the padding's x reads on the row's own lines
 	}
@@ -256,16 +275,20 @@ __global__ void __launch_bounds__(kBlock) window_ranges(const int32_t *__restric
 	}
 }
 
-int sell_convert(spmvhip_format *f, bool sorted)
+int sell_convert(spmvhip_format *f, SellOrder ord)
 {
+	const bool sorted = ord != kSellUnsorted;
 	hipStream_t s = f->stream;
 	const long m = f->m;
 	if (f->p.C != 0 && f->p.C != kSliceC)
 		return set_error(SPMVHIP_EINVAL, "sell: C = %d unsupported (slice height is one 64-lane wavefront)", f->p.C);
 	long sigma = f->p.sigma > 0 ? f->p.sigma : 4096;
 	if (sorted && sigma > kMaxSigma)
-		return set_error(SPMVHIP_EINVAL, "sell_c_s: sigma = %ld > %d (window must fit one workgroup's LDS)",
-				sigma, kMaxSigma);
+		return set_error(SPMVHIP_EINVAL, "%s: sigma = %ld > %d (window must fit one workgroup's LDS)",
+				f->name, sigma, kMaxSigma);
+	if (ord == kSellAscending && sigma % kSliceC != 0)
+		return set_error(SPMVHIP_EINVAL, "sell_sorted: the row window sigma = %ld must be a multiple of %d "
+				"(whole slices, as the reference's thread ranges)", sigma, kSliceC);
 	const int T = f->p.lanes_per_row > 0 ? f->p.lanes_per_row : 1;
 	if (T > kSliceC || (T & (T - 1)))
 		return set_error(SPMVHIP_EINVAL, "sell: lanes_per_row %d is not a power of two in [1, 64]", T);
@@ -293,7 +316,7 @@ int sell_convert(spmvhip_format *f, bool sorted)
 		SPMVHIP_CHECK(hipFuncSetAttribute((const void *) sigma_window_sort,
 				hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
 		sigma_window_sort<<<(unsigned) nwin, kSortThreads, lds, s>>>(len.as<int32_t>(), m, (int) sigma,
-				f->row_order.as<int32_t>());
+				ord == kSellAscending, f->row_order.as<int32_t>());
 		SPMVHIP_CHECK(hipGetLastError());
 		order = f->row_order.as<int32_t>();
 		if (ym == 3) {
@@ -328,7 +351,8 @@ int sell_convert(spmvhip_format *f, bool sorted)
 	SPMVHIP_TRY(f->s_vals.alloc(total * sizeof(double)));
 	sell_fill<<<grid_for(f->nslices, kWavesPerBlock), kBlock, 0, s>>>(f->row_ptr.as<int32_t>(), f->col.as<int32_t>(),
 			f->val.as<double>(), order, sorted ? nullptr : last_ne.as<int64_t>(), f->slice_w.as<int32_t>(),
-			f->slice_ptr.as<int64_t>(), m, f->nslices, T, f->s_cols.as<int32_t>(), f->s_vals.as<double>());
+			f->slice_ptr.as<int64_t>(), m, f->nslices, T, ord == kSellAscending, f->s_cols.as<int32_t>(),
+			f->s_vals.as<double>());
 	SPMVHIP_CHECK(hipGetLastError());
 	f->x_cap = 0;
 	if (f->p.x_window > 0) {
@@ -422,7 +446,7 @@ int sell_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 		return SPMVHIP_OK;
 	const int remap = xcd_chunk(f->p.xcd_remap);
 	const unsigned grid = xcd_padded_grid(grid_for(f->nslices, kWavesPerBlock), remap);
-	const int32_t *order = f->kind == SPMVHIP_SELL_C_S ? f->row_order.as<int32_t>() : nullptr;
+	const int32_t *order = f->row_order.p ? f->row_order.as<int32_t>() : nullptr;
 	int u = f->p.unroll > 0 ? f->p.unroll : 8;
 	// y_mode 0 (auto): through LDS when the permutation cannot leave a
 	// workgroup's 256 rows (sigma divides 256) or several lanes per row would

@@ -1,6 +1,6 @@
 // HIP-backed Matrix_Format subclass: the drop-in replacement of one of the
 // reference's format translation units (src/spmv_kernels/csr.cpp,
-// csr_vec.cpp, sell.cpp, sell_c_s.cpp, DIV/div.cpp). One object file per
+// csr_vec.cpp, sell.cpp, sell_c_s.cpp, sell_sorted.cpp, DIV/div.cpp). One object file per
 // format, selected at compile time with -DSPMVHIP_FORMAT_KIND=<enum
 // spmvhip_kind>, exactly as the reference links one format per executable
 // (src/Makefile_in:47-74). Link it with the reference's bench.o/bench_spmv.o
@@ -13,9 +13,12 @@
 //                     stay on the GPU; y is copied back by statistics_print_data()
 //                     and when the object is destroyed
 //   SELLCS_C, SELLCS_SIGMA          SELL-C-sigma slice height / sort window
+//                                   (SELLCS_SIGMA is SELL_SORTED's row window too)
 //   CSRV_BLOCK_NNZ, CSRV_LANES, CSRV_SPLIT_NNZ, CSRV_NARROW_LOADS   CSR-vector row-block tuning
 //   SPMV_XCD_REMAP, SELL_UNROLL, SELL_PIPELINE, SELL_X_WINDOW, SELL_Y_MODE
 //   DIV_THREADS, DIV_LANES, CSRCV_NUM_PACKET_VALS, DIV_LAYOUT   DIV-RF codec / device layout
+//   DIV_PACKET_REORDERING=1          packet groups in median-column order (config.sh's variable)
+//   DIV_VC_TOLERANCE=t               lossy DIV-RF values, relative error <= t (config.sh's variable)
 
 #include <cstdio>
 #include <cstdlib>
@@ -117,6 +120,9 @@ struct Matrix_Format *csr_to_format(INT_T *row_ptr, INT_T *col_ind, ValueTypeRef
 	p.div_acc_copies = (int) env_long("DIV_ACC_COPIES", 0);
 	p.narrow_loads = (int) env_long("CSRV_NARROW_LOADS", 0);
 	p.div_layout = (int) env_long("DIV_LAYOUT", 0);
+	p.div_reorder = (int) env_long("DIV_PACKET_REORDERING", 0);
+	if (const char *tol = getenv("DIV_VC_TOLERANCE"))
+		p.div_tolerance = atof(tol);
 	p.div_packet_vals = env_long("CSRCV_NUM_PACKET_VALS", 0);
 
 	HipMatrixFormat *f = new HipMatrixFormat(m, n, nnz);

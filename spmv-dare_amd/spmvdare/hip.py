@@ -19,7 +19,7 @@ from . import _native
 
 _lib = _native.load("spmvhip")
 
-KINDS = {"csr_naive": 0, "csr_vec": 1, "sell": 2, "sell_c_s": 3, "div_rf": 4}
+KINDS = {"csr_naive": 0, "csr_vec": 1, "sell": 2, "sell_c_s": 3, "div_rf": 4, "sell_sorted": 5}
 KIND_NAMES = {v: k for k, v in KINDS.items()}
 
 
@@ -31,7 +31,8 @@ class Params(ctypes.Structure):
                 ("div_threads", ctypes.c_int), ("div_lanes", ctypes.c_int),
                 ("div_packet_vals", ctypes.c_long), ("pipeline", ctypes.c_int),
                 ("y_mode", ctypes.c_int), ("div_acc_copies", ctypes.c_int), ("narrow_loads", ctypes.c_int),
-                ("div_layout", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
+                ("div_layout", ctypes.c_int), ("div_reorder", ctypes.c_int),
+                ("div_tolerance", ctypes.c_double), ("reserved", ctypes.c_int * 2)]
 
 
 class Info(ctypes.Structure):
@@ -99,6 +100,9 @@ _sig = {
     "spmvhip_div_encode": ([_vp, _vp, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_int,
                             ctypes.c_int, ctypes.c_long, _vp, ctypes.POINTER(ctypes.c_size_t), _vp,
                             ctypes.POINTER(ctypes.c_long), _vp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+    "spmvhip_div_reduce_precision": ([_vp, ctypes.c_long, ctypes.c_double, _vp], ctypes.c_int),
+    "spmvhip_div_group_order": ([_vp, _vp, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_long, _vp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
     "spmvhip_div_device_encode": ([_vp, _vp, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_long, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_size_t),
                                    _vp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
@@ -446,6 +450,31 @@ def div_encode(row_ptr, col_ind, values, n, T=1, L=64, P=16384):
                                    b.ctypes.data, ctypes.byref(nb), off.ctypes.data, ctypes.byref(npk),
                                    grp.ctypes.data, ctypes.byref(ng)), "div_encode")
     return b[:nb.value], off, grp
+
+
+def div_group_order(row_ptr, col_ind, values, n, T=1, L=64, P=16384):
+    """The packet-group order of the reference's packet reordering for the
+    stream div_encode returns (no device needed)."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    ci = np.ascontiguousarray(col_ind, dtype=np.int32)
+    va = np.ascontiguousarray(values, dtype=np.float64)
+    m, nnz = rp.size - 1, ci.size
+    ng = ctypes.c_long()
+    _check(_lib.spmvhip_div_group_order(rp.ctypes.data, ci.ctypes.data, va.ctypes.data, m, n, nnz, T, L, P, None,
+                                        ctypes.byref(ng)), "div_group_order")
+    order = np.empty(max(ng.value, 1), dtype=np.int32)
+    _check(_lib.spmvhip_div_group_order(rp.ctypes.data, ci.ctypes.data, va.ctypes.data, m, n, nnz, T, L, P,
+                                        order.ctypes.data, ctypes.byref(ng)), "div_group_order")
+    return order[:ng.value]
+
+
+def div_reduce_precision(values, tolerance):
+    """DIV_RF's lossy value rounding (no device needed)."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    out = np.empty_like(v)
+    _check(_lib.spmvhip_div_reduce_precision(v.ctypes.data, v.size, float(tolerance), out.ctypes.data),
+           "div_reduce_precision")
+    return out
 
 
 def statistics_print_labels():

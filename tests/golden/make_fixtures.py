@@ -7,12 +7,22 @@ reference's generator (artificial-matrix-generator/artificial_matrix_generation.
 and SELL-C-sigma conversion (src/spmv_kernels/sell-C-s/radix_sort.c,
 sellcs_format.c).
 
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py [sell_sorted|div_reorder|div_lossy]   (one set only)
 
 Outputs (tests/golden/):
     generator.json  per parameter line: m, n, nnz, FNV-1a 64 of row_ptr /
                     col_ind / raw values, and the reference's descriptor stats
     sellcs.json     per (matrix, C, sigma): FNV-1a 64 of each converted array
+    sell_sorted.json  per (matrix, threads): the reference's SELL_SORTED
+                    (src/spmv_kernels/sell_sorted.cpp, AVX2 build: 4-row
+                    clusters): thread windows and digests of its permutation,
+                    cluster sizes, ja, a and of y for x_j = ((j % 13) + 1)/8 - 1
+    div_lossy.json  per (matrix, T, P, tolerance): the reference's DIV-RF
+                    stream with DIV_VC_TOLERANCE (digest, header padding
+                    masked); tiny_div_lossy.npz: one such stream in full
+    div_reorder.json  per (matrix, T, P): the reference's DIV-RF stream with
+                    DIV_PACKET_REORDERING=1, as the sequence of its packet
+                    groups (row_min of each group's first packet) per partition
     tiny_csr.npz, tiny_sellcs.npz   full arrays of the 1000 x 1000 test matrix
 """
 import json
@@ -166,6 +176,9 @@ def main():
     with open(os.path.join(HERE, "sellcs.json"), "w") as f:
         json.dump(sell, f, indent=1, sort_keys=True)
     div_fixtures(gen)
+    div_reorder_fixtures(gen)
+    div_lossy_fixtures(gen)
+    sell_sorted_fixtures(gen)
 
 
 def read_div(path):
@@ -231,5 +244,92 @@ def div_fixtures(gen):
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+def div_lossy_fixtures(gen):
+    if not os.path.exists(os.path.join(REF, "div_dump")):
+        sys.exit("oracle/_ref/div_dump missing: run `make -C oracle ref` first")
+    cases = [("test_txt", 4, 512, "1e-6"), ("cfg1", 8, 4096, "1e-3"), ("batch_03", 5, 1000, "1e-9"),
+             ("neigh_two", 2, 16384, "0.01"), ("dense_rows", 1, 2048, "1e-4"), ("tall", 3, 16384, "0.3")]
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        for name, T, P, tol in cases:
+            line = gen[name]["line"]
+            csr = os.path.join(td, "c.bin")
+            run_gen(line, csr)
+            dump = os.path.join(td, "d.bin")
+            subprocess.run([os.path.join(REF, "div_dump"), csr, str(P), dump, "0", tol], check=True,
+                           capture_output=True, env=dict(os.environ, OMP_NUM_THREADS=str(T)))
+            parts = read_div(dump)
+            stream = np.concatenate([mask_padding(b) for b, _ in parts]) if parts else np.zeros(0, np.uint8)
+            key = "%s/T%d/P%d/tol%s" % (name, T, P, tol)
+            out[key] = {"line": line, "T": T, "L": 4, "P": P, "tolerance": float(tol), "bytes": int(stream.size),
+                        "stream": digest(stream)}
+            print(key, stream.size, flush=True)
+            if name == "test_txt":
+                np.savez_compressed(os.path.join(HERE, "tiny_div_lossy.npz"), stream=stream)
+    with open(os.path.join(HERE, "div_lossy.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
+def div_reorder_fixtures(gen):
+    if not os.path.exists(os.path.join(REF, "div_dump")):
+        sys.exit("oracle/_ref/div_dump missing: run `make -C oracle ref` first")
+    cases = [("test_txt", 1, 97), ("test_txt", 4, 512), ("cfg1", 8, 4096), ("neigh_two", 2, 2048),
+             ("batch_03", 5, 1000), ("tall", 4, 16384), ("simple_placement", 2, 300), ("stream_01", 3, 4096)]
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        for name, T, P in cases:
+            line = gen[name]["line"]
+            csr = os.path.join(td, "c.bin")
+            run_gen(line, csr)
+            dump = os.path.join(td, "d.bin")
+            subprocess.run([os.path.join(REF, "div_dump"), csr, str(P), dump, "1"], check=True,
+                           capture_output=True, env=dict(os.environ, OMP_NUM_THREADS=str(T)))
+            seq = []
+            for b, pos in read_div(dump):
+                seq.append([int(np.frombuffer(b[q + 12:q + 16].tobytes(), dtype="<u4")[0]) for q in pos[:-1]])
+            key = "%s/T%d/P%d" % (name, T, P)
+            out[key] = {"line": line, "T": T, "P": P, "group_first_rows": seq}
+            print(key, sum(len(s) for s in seq), flush=True)
+    with open(os.path.join(HERE, "div_reorder.json"), "w") as f:
+        json.dump(out, f, sort_keys=True)
+
+
+def sell_sorted_fixtures(gen):
+    if not os.path.exists(os.path.join(REF, "sell_sorted_dump")):
+        sys.exit("oracle/_ref/sell_sorted_dump missing: run `make -C oracle ref` first")
+    cases = [("test_txt", 1), ("test_txt", 3), ("test_txt", 8), ("cfg1", 4), ("neigh_two", 2),
+             ("batch_03", 5), ("dense_rows", 1), ("tall", 4), ("stream_01", 7)]
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        for name, T in cases:
+            line = gen[name]["line"]
+            csr = os.path.join(td, "c.bin")
+            run_gen(line, csr)
+            dump = os.path.join(td, "s.bin")
+            subprocess.run([os.path.join(REF, "sell_sorted_dump"), csr, dump], check=True, capture_output=True,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(T)))
+            m = gen[name]["m"]
+            with open(dump, "rb") as f:
+                VL, TT, ncl, ne = struct.unpack("<4q", f.read(32))
+                rng = np.frombuffer(f.read(16 * TT), dtype=np.int64).reshape(TT, 2)
+                cptr = np.frombuffer(f.read(4 * (ncl + 1)), dtype=np.int32)
+                perm = np.frombuffer(f.read(4 * m), dtype=np.int32)
+                ja = np.frombuffer(f.read(4 * ne), dtype=np.int32)
+                a = np.frombuffer(f.read(8 * ne), dtype=np.float64)
+                y = np.frombuffer(f.read(8 * m), dtype=np.float64)
+            key = "%s/T%d" % (name, T)
+            out[key] = {"line": line, "VL": int(VL), "T": T, "nnz_ext": int(ne),
+                        "windows": [int(v) for v in rng[:, 0]] + [int(rng[-1, 1])],
+                        "perm": digest(perm), "cluster_ptr": digest(cptr.astype(np.int64)),
+                        "ja": digest(ja.astype(np.int64)), "a": digest(a), "y": digest(y)}
+            print(key, int(ne), flush=True)
+    with open(os.path.join(HERE, "sell_sorted.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] in (["sell_sorted"], ["div_reorder"], ["div_lossy"]):
+        with open(os.path.join(HERE, "generator.json")) as f:
+            globals()[sys.argv[1] + "_fixtures"](json.load(f))
+    else:
+        main()

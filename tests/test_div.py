@@ -17,6 +17,8 @@ import pyoracle
 HERE = os.path.dirname(os.path.abspath(__file__))
 DIV = json.load(open(os.path.join(HERE, "golden", "div.json")))
 GEN = json.load(open(os.path.join(HERE, "golden", "generator.json")))
+REORDER = json.load(open(os.path.join(HERE, "golden", "div_reorder.json")))
+LOSSY = json.load(open(os.path.join(HERE, "golden", "div_lossy.json")))
 
 
 def digest(a):
@@ -31,6 +33,121 @@ def test_encoder_matches_reference_stream(key):
     assert b.size == f["bytes"]
     assert digest(b) == f["stream"]  # the encoder writes the header padding as 0
     assert grp.size - 1 == sum(f["groups_per_partition"])
+
+
+# ------------------------------------------------------------ lossy values
+
+def test_lossy_oracle_matches_reference_decode(golden_dir):
+    """The reference's own lossy stream (test_txt, T=4, P=512, tolerance
+    1e-6) decodes to the oracle's rounding of every value."""
+    a = gen.generate(gen.GenParams.parse(GEN["test_txt"]["line"]), stats=False, map_values=False)
+    stream = np.load(os.path.join(golden_dir, "tiny_div_lossy.npz"))["stream"]
+    rows, cols, vals = pyoracle.div_decode(stream, 4, a.nnz)
+    assert rows.size == a.nnz
+    # every (row, col) is unique, so it names its CSR entry
+    csr_index = {(int(r), int(c)): j for r in range(a.m) for j, c in
+                 zip(range(a.row_ptr[r], a.row_ptr[r + 1]), a.col_ind[a.row_ptr[r]:a.row_ptr[r + 1]])}
+    order = np.array([csr_index[(int(r), int(c))] for r, c in zip(rows, cols)])
+    expect = pyoracle.div_reduce_precision(a.values[order], 1e-6)
+    np.testing.assert_array_equal(vals, expect)
+    assert (vals != a.values[order]).any()
+
+
+@pytest.mark.parametrize("key", sorted(LOSSY))
+def test_lossy_encoder_matches_reference_stream(key):
+    """The product's rounding equals the oracle's bit for bit, and encoding
+    the rounded values gives the reference's lossy stream byte for byte."""
+    f = LOSSY[key]
+    a = gen.generate(gen.GenParams.parse(f["line"]), stats=False, map_values=False)
+    r = hip.div_reduce_precision(a.values, f["tolerance"])
+    sample = np.random.default_rng(0).choice(a.nnz, size=min(a.nnz, 20000), replace=False)
+    np.testing.assert_array_equal(r[sample], pyoracle.div_reduce_precision(a.values[sample], f["tolerance"]))
+    b, off, grp = hip.div_encode(a.row_ptr, a.col_ind, r, a.n, T=f["T"], L=f["L"], P=f["P"])
+    assert b.size == f["bytes"]
+    assert digest(b) == f["stream"]
+
+
+@pytest.mark.parametrize("tol", [0.3, 1e-2, 1e-6, 2.0 ** -30, 1e-12, 1e-15])
+def test_lossy_rounding_bounds(tol):
+    """Error bound and bit pattern of the rounding on values of every
+    magnitude and sign: |r - v| <= tol |v|, the low 52 - b fraction bits of r
+    are zero (b = ceil(|log2 tol|)), and r is one of the two neighbours of v
+    on that grid."""
+    rng = np.random.default_rng(7)
+    v = np.concatenate([rng.standard_normal(5000) * 10.0 ** rng.integers(-300, 300, 5000),
+                        [0.0, -0.0, 1.0, -2.5, 0.1, 1 / 3, np.finfo(float).tiny, np.finfo(float).max / 4]])
+    r = hip.div_reduce_precision(v, tol)
+    np.testing.assert_array_equal(r, pyoracle.div_reduce_precision(v, tol))
+    nz = v != 0
+    assert (np.abs(r[nz] - v[nz]) <= tol * np.abs(v[nz])).all()
+    keep = int(np.ceil(abs(np.log2(tol))))
+    u, ru = v.view(np.uint64), r.view(np.uint64)
+    if keep < 52:
+        low = np.uint64((1 << (52 - keep)) - 1)
+        assert ((ru & low) == 0).all()
+        dn, up = u & ~low, (u + low) & ~low
+        assert ((ru == dn) | (ru == up) | (u == 0) | (v == 0)).all()
+    else:
+        np.testing.assert_array_equal(r, v)
+    np.testing.assert_array_equal(hip.div_reduce_precision(v, 0.0), v)
+
+
+def test_lossy_tolerance_rejected():
+    with pytest.raises(hip.SpmvHipError, match="tolerance"):
+        hip.div_reduce_precision(np.ones(3), 1.0)
+
+
+def median_key(cols):
+    """The reference's median (lib/array_metrics.c:940-1000, method
+    "median_unbiased" at q = 1/2): virtual index (N - 1) q + q / 3 + 1 / 3 =
+    N / 2 into the sorted values, linear between neighbours, the maximum when
+    the index reaches the last value -- half a position above numpy's
+    median_unbiased, which puts it at N / 2 - 1/2. Truncated to an integer."""
+    v = np.sort(np.asarray(cols, dtype=np.float64))
+    N = v.size
+    if N // 2 >= N - 1:
+        return int(v[-1])
+    if N % 2 == 0:
+        return int(v[N // 2])
+    return int(0.5 * v[N // 2] + 0.5 * v[N // 2 + 1])
+
+
+@pytest.mark.parametrize("key", sorted(REORDER))
+def test_packet_reordering_matches_reference(key):
+    """The group order of DIV_PACKET_REORDERING=1 (div.cpp:377-395,455-498)
+    against the reference's own reordered streams. Each group is named by the
+    first row of its first packet. The key of the partition's k-th group is
+    the median of the columns of the partition's k-th packet as median_key
+    states it: the reference's sequence must be sorted by it, and the
+    encoder's order must equal the reference's up to the order of equal keys
+    (the reference sorts with an unstable quicksort; the encoder keeps the
+    stream order among them)."""
+    f = REORDER[key]
+    a = gen.generate(gen.GenParams.parse(f["line"]), stats=False, map_values=False)
+    b, off, grp = hip.div_encode(a.row_ptr, a.col_ind, a.values, a.n, T=f["T"], L=4, P=f["P"])
+    order = hip.div_group_order(a.row_ptr, a.col_ind, a.values, a.n, T=f["T"], L=4, P=f["P"])
+    ng = grp.size - 1
+    assert sorted(order.tolist()) == list(range(ng))
+    nv = np.array([int(np.frombuffer(b[o:o + 4].tobytes(), "<u4")[0]) for o in off[:-1]], dtype=np.int64)
+    first_nnz = np.concatenate([[0], np.cumsum(nv)])
+    row_min = np.array([int(np.frombuffer(b[off[p] + 12:off[p] + 16].tobytes(), "<u4")[0]) for p in grp[:-1]])
+    g0 = 0
+    for seq in f["group_first_rows"]:
+        g1 = g0 + len(seq)
+        keys = {}
+        for k in range(g1 - g0):
+            p = grp[g0] + k  # the partition's k-th packet
+            cols = a.col_ind[first_nnz[p]:first_nnz[p + 1]]
+            keys[int(row_min[g0 + k])] = median_key(cols)
+        ref_keys = [keys[r] for r in seq]
+        assert ref_keys == sorted(ref_keys), "the reference's order is not sorted by the median key"
+        mine = [int(row_min[g]) for g in order[g0:g1]]
+        assert [keys[r] for r in mine] == ref_keys
+        natural = [int(r) for r in row_min[g0:g1]]
+        assert mine == sorted(natural, key=lambda r: keys[r])  # stable: stream order among equal keys
+        assert sorted(mine) == sorted(seq)
+        g0 = g1
+    assert g0 == ng
 
 
 def test_tiny_stream_and_groups(golden_dir):

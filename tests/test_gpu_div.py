@@ -144,3 +144,56 @@ def test_div_layout_rejected():
     for bad in (-1, 5):
         with pytest.raises(hip.SpmvHipError):
             hip.csr_to_format(rp, ci, va, rp.size - 1, n, ci.size, kind="div_rf", div_layout=bad)
+
+
+@pytest.mark.parametrize("name", ["test_txt", "cfg1", "batch_03", "ragged_4097", "huge_rows", "dense_rows",
+                                  "one_row_empty", "wide_diag_2m"])
+@pytest.mark.parametrize("T,P,layout", [(1, 16384, 0), (1, 16384, 1), (4, 512, 1), (4, 512, 2), (4, 512, 3),
+                                        (3, 300, 0), (3, 300, 4)])
+def test_div_packet_reordering(name, T, P, layout):
+    """DIV_PACKET_REORDERING=1: the groups run in the encoder's median-column
+    order (checked against the reference's reordered streams on the CPU);
+    every group owns its rows, so y is bit-identical to stream order."""
+    rp, ci, va, n = wide_or_matrix(name)
+    m = rp.size - 1
+    print("case reorder", name, layout, T, P, flush=True)
+    x = gen.make_x(n, 91)
+    kw = dict(div_threads=T, div_packet_vals=P, div_layout=layout, unroll=8)
+    f0 = hip.csr_to_format(rp, ci, va, m, n, ci.size, kind="div_rf", **kw)
+    f1 = hip.csr_to_format(rp, ci, va, m, n, ci.size, kind="div_rf", div_reorder=1, **kw)
+    y0, y1 = run(f0, x, m), run(f1, x, m)
+    np.testing.assert_array_equal(y1, y0)
+    check_dd(y1, rp, ci, va, x, "div_rf/reorder/T%d/P%d/layout%d/%s" % (T, P, layout, name))
+
+
+def wide_or_matrix(name):
+    if name in WIDE:
+        a = gen.generate(gen.GenParams.parse(WIDE[name]), stats=False)
+        return a.row_ptr, a.col_ind, a.values, a.n
+    return matrix(name)
+
+
+@pytest.mark.parametrize("name", ["test_txt", "cfg1", "batch_03", "dense_rows", "huge_rows", "one_row_empty"])
+@pytest.mark.parametrize("tol", [1e-3, 1e-6, 1e-9])
+@pytest.mark.parametrize("layout", [0, 1])
+def test_div_lossy_values(name, tol, layout):
+    """DIV_VC_TOLERANCE: the device y is the product of the rounded values
+    (within 1e-13 normwise of the oracle's csr_naive over the oracle's
+    rounding, which is pinned to the reference's lossy streams), and within
+    the tolerance of the exact product."""
+    rp, ci, va, n = wide_or_matrix(name)
+    m = rp.size - 1
+    print("case lossy", name, tol, layout, flush=True)
+    x = gen.make_x(n, 89)
+    f = hip.csr_to_format(rp, ci, va, m, n, ci.size, kind="div_rf", div_tolerance=tol, div_layout=layout,
+                          div_packet_vals=4096)
+    y = run(f, x, m)
+    vr = pyoracle.div_reduce_precision(va, tol)
+    y_ref = pyoracle.csr_naive(rp, ci, vr, x)
+    err = pyoracle.normwise_rel_err(y, y_ref)
+    print("lossy %s tol=%g: normwise vs rounded-value csr_naive %.3e" % (name, tol, err), flush=True)
+    assert err <= TOL
+    # |y - A x| <= tol |A| |x| entrywise (every value moved by at most tol of itself)
+    y_exact = pyoracle.csr_naive(rp, ci, va, x)
+    y_abs = pyoracle.csr_naive(rp, ci, np.abs(va), np.abs(x))
+    assert np.linalg.norm(y - y_exact) <= tol * np.linalg.norm(y_abs) * 1.001 + 1e-300

@@ -17,7 +17,7 @@ ENV = dict(os.environ, USE_ARTIFICIAL_MATRICES="1", USE_PROCESSES="0", GPU_KERNE
            OMP_NUM_THREADS="4")
 
 
-@pytest.mark.parametrize("fmt", ["csr_naive", "csr_vec", "sell", "sell_c_s", "div_rf"])
+@pytest.mark.parametrize("fmt", ["csr_naive", "csr_vec", "sell", "sell_c_s", "div_rf", "sell_sorted"])
 def test_own_driver(fmt):
     exe = os.path.join(ROOT, "spmv-dare_amd", "bin", "spmv_hip_%s_d.exe" % fmt)
     r = subprocess.run([exe] + LINE, env=ENV, capture_output=True, text=True, timeout=120)
@@ -29,7 +29,7 @@ def test_own_driver(fmt):
     assert csv[18].startswith("HIP_")
 
 
-@pytest.mark.parametrize("fmt", ["csr_vec", "sell_c_s", "div_rf"])
+@pytest.mark.parametrize("fmt", ["csr_vec", "sell_c_s", "div_rf", "sell_sorted"])
 def test_reference_driver(fmt):
     exe = os.path.join(ROOT, "oracle", "_ref", "spmv_hip_%s_d.exe" % fmt)
     if not os.path.exists(exe):
@@ -48,6 +48,12 @@ MTX_DIR = os.path.join(ROOT, "tests", "golden", "mtx")
 MTX_FILES = ["general_real", "symmetric_real", "skew_real", "symmetric_pattern", "general_integer",
              "general_complex", "hermitian_complex", "no_banner", "empty_rows"]
 ENV_MTX = dict(ENV, USE_ARTIFICIAL_MATRICES="0")
+# The reference's driver reads y for check_accuracy right after its timing
+# loop (bench_spmv.cpp:434) with no call into the format before it, so its
+# validation runs use GPU_KERNEL=0: the drop-in copies y back after every call
+# (with GPU_KERNEL=1, y reaches the host at statistics_print_data and at
+# destruction, which this driver calls only after the check).
+ENV_MTX_REF = dict(ENV_MTX, GPU_KERNEL="0")
 
 
 def accuracy_line(out):
@@ -57,7 +63,7 @@ def accuracy_line(out):
     return {k: float(v) for k, v in vals.items()}
 
 
-@pytest.mark.parametrize("fmt", ["csr_naive", "csr_vec", "sell", "sell_c_s", "div_rf"])
+@pytest.mark.parametrize("fmt", ["csr_naive", "csr_vec", "sell", "sell_c_s", "div_rf", "sell_sorted"])
 @pytest.mark.parametrize("name", MTX_FILES)
 def test_own_driver_matrix_market(fmt, name):
     exe = os.path.join(ROOT, "spmv-dare_amd", "bin", "spmv_hip_%s_d.exe" % fmt)
@@ -71,7 +77,7 @@ def test_own_driver_matrix_market(fmt, name):
     assert acc["max_ae"] <= 1e-12 and acc["mape"] <= 1e-8
 
 
-@pytest.mark.parametrize("fmt", ["csr_vec", "sell_c_s", "div_rf"])
+@pytest.mark.parametrize("fmt", ["csr_vec", "sell_c_s", "div_rf", "sell_sorted"])
 @pytest.mark.parametrize("name", MTX_FILES)
 def test_reference_driver_matrix_market(fmt, name):
     """The reference's unmodified driver reads the file with its own import
@@ -81,7 +87,7 @@ def test_reference_driver_matrix_market(fmt, name):
         pytest.skip("reference driver not built here (needs the reference sources at build time)")
     path = os.path.join(MTX_DIR, name + ".mtx")
     print("case mtx reference driver", fmt, name, flush=True)
-    r = subprocess.run([exe, path], env=ENV_MTX, capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, path], env=ENV_MTX_REF, capture_output=True, text=True, timeout=120)
     print(r.stdout[-600:], r.stderr[-300:])
     assert r.returncode == 0, r.stdout[-800:] + r.stderr[-400:]
     assert "Test failed" not in r.stdout
@@ -89,7 +95,7 @@ def test_reference_driver_matrix_market(fmt, name):
     assert acc["max_ae"] <= 1e-12
 
 
-@pytest.mark.parametrize("fmt", ["csr_naive", "csr_vec", "sell_c_s", "div_rf"])
+@pytest.mark.parametrize("fmt", ["csr_naive", "csr_vec", "sell_c_s", "div_rf", "sell_sorted"])
 @pytest.mark.parametrize("name", ["general_real", "symmetric_real", "general_complex", "empty_rows"])
 def test_reference_cpu_formats_matrix_market(fmt, name):
     """The CPU baseline's executables (the reference's formats and driver;

@@ -2,8 +2,8 @@
 
 Bars (stated per check):
   csr_naive        y bit-identical to the oracle's sequential multiply-then-add
-  sell, sell_c_s   converted arrays bit-identical to the oracle's conversion
-                   (which is pinned to the reference's own conversion by the
+  sell, sell_c_s,  converted arrays bit-identical to the oracle's conversion
+  sell_sorted      (which is pinned to the reference's own conversion by the
                    golden fixtures) and y bit-identical to the oracle's
                    lane-sequential fused multiply-add
   csr_vec          normwise relative error <= 1e-13 against csr_naive
@@ -160,6 +160,37 @@ def test_sell_c_sigma_bit_exact(name, sigma):
     y = run(f, x, m)
     np.testing.assert_array_equal(y, pyoracle.sellcs_spmv(ref, 64, x))
     check_dd(y, rp, ci, va, x, "sell_c_s/s%d/%s" % (sigma, name))
+
+
+@pytest.mark.parametrize("name", ALL)
+@pytest.mark.parametrize("sigma,y_mode", [(64, 0), (256, 0), (4096, 0), (16384, 0), (4096, 3), (256, 1)])
+def test_sell_sorted_bit_exact(name, sigma, y_mode):
+    """SELL_SORTED (sell_sorted.cpp) with windows of sigma rows: the
+    permutation, slices (including the padding columns) and y bit-identical
+    to the oracle over the same windows (the oracle is pinned to the
+    reference's own build over its thread windows)."""
+    rp, ci, va, n = matrix(name)
+    m = rp.size - 1
+    print("case sell_sorted", name, sigma, y_mode, flush=True)
+    x = gen.make_x(n, 95)
+    f = hip.csr_to_format(rp, ci, va, m, n, ci.size, kind="sell_sorted", sigma=sigma, y_mode=y_mode)
+    assert f.info()["format_name"] == "HIP_SELL_SORTED"
+    ref = pyoracle.sellsorted_convert(rp, ci, va, 64, list(range(0, m, sigma)) + [m])
+    if m > 0:
+        np.testing.assert_array_equal(f.export(0, np.int32), ref["order"])
+    np.testing.assert_array_equal(f.export(2, np.int64), ref["cluster_ptr"])
+    np.testing.assert_array_equal(f.export(1, np.int32), np.diff(ref["cluster_ptr"]) // 64)
+    np.testing.assert_array_equal(f.export(3, np.int32), ref["ja"])
+    np.testing.assert_array_equal(f.export(4, np.float64), ref["a"])
+    y = run(f, x, m)
+    np.testing.assert_array_equal(y, pyoracle.sellsorted_spmv(ref, 64, x))
+    check_dd(y, rp, ci, va, x, "sell_sorted/s%d/%s" % (sigma, name))
+
+
+def test_sell_sorted_window_must_be_whole_slices():
+    rp, ci, va, n = matrix("test_txt")
+    with pytest.raises(hip.SpmvHipError, match="multiple of 64"):
+        hip.csr_to_format(rp, ci, va, rp.size - 1, n, ci.size, kind="sell_sorted", sigma=100)
 
 
 @pytest.mark.parametrize("name", ["test_txt", "cfg1", "ragged_4097", "huge_rows"])

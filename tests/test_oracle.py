@@ -15,6 +15,7 @@ import pyoracle
 HERE = os.path.dirname(os.path.abspath(__file__))
 GEN = json.load(open(os.path.join(HERE, "golden", "generator.json")))
 SELL = json.load(open(os.path.join(HERE, "golden", "sellcs.json")))
+SSORT = json.load(open(os.path.join(HERE, "golden", "sell_sorted.json")))
 
 
 def digest(a):
@@ -83,6 +84,47 @@ def test_sellcs_spmv_oracle_against_dense():
     for i in range(300):
         dense[i, ci[rp[i]:rp[i + 1]]] = va[rp[i]:rp[i + 1]]
     np.testing.assert_allclose(y, dense @ x, rtol=1e-12, atol=1e-13)
+
+
+# ------------------------------------------------------------ SELL_SORTED
+
+@pytest.mark.parametrize("key", sorted(SSORT))
+def test_sellsorted_oracle_matches_reference(key):
+    """Thread windows, permutation, cluster sizes, ja, a and y of the
+    reference's own SELL_SORTED build (sell_sorted.cpp, 4-row clusters)."""
+    f = SSORT[key]
+    a = gen.generate(gen.GenParams.parse(f["line"]), stats=False, map_values=False)
+    win = pyoracle.sellsorted_windows(a.row_ptr, f["VL"], f["T"])
+    assert win.tolist() == f["windows"]
+    conv = pyoracle.sellsorted_convert(a.row_ptr, a.col_ind, a.values, f["VL"], win)
+    assert int(conv["cluster_ptr"][-1]) == f["nnz_ext"]
+    for k in ("perm", "cluster_ptr", "ja", "a"):
+        assert digest(conv[k]) == f[k], k
+    x = ((np.arange(a.n) % 13) + 1) / 8.0 - 1.0
+    assert digest(pyoracle.sellsorted_spmv(conv, f["VL"], x)) == f["y"]
+
+
+@pytest.mark.parametrize("VL,bounds", [(4, [0, 12, 40, 101]), (64, [0, 128, 517]), (8, [0, 517])])
+def test_sellsorted_order_brute_force(VL, bounds):
+    rng = np.random.default_rng(VL)
+    rp, ci, va = random_csr(rng, 517, 300, 12)
+    if bounds[-1] != 517:
+        rp = rp[:bounds[-1] + 1]
+    m = rp.size - 1
+    lens = np.diff(rp)
+    conv = pyoracle.sellsorted_convert(rp, ci, va, VL, bounds)
+    order = []
+    for w0, w1 in zip(bounds[:-1], bounds[1:]):
+        order += sorted(range(w0, w1), key=lambda i: (lens[i], i))
+    np.testing.assert_array_equal(conv["order"], order)
+    for c in range((m + VL - 1) // VL):
+        rows = conv["order"][c * VL:(c + 1) * VL]
+        assert conv["cluster_ptr"][c + 1] - conv["cluster_ptr"][c] == VL * lens[rows].max()
+    x = rng.uniform(-1, 1, 300)
+    dense = np.zeros((m, 300))
+    for i in range(m):
+        dense[i, ci[rp[i]:rp[i + 1]]] = va[rp[i]:rp[i + 1]]
+    np.testing.assert_allclose(pyoracle.sellsorted_spmv(conv, VL, x), dense @ x, rtol=1e-12, atol=1e-13)
 
 
 # ------------------------------------------------------------ SELL
