@@ -1,7 +1,14 @@
-"""Build a selector corpus: generated matrices x the 40 tuning variants, each
+"""Build a selector corpus: generated matrices x the tuning variants, each
 timed on the GPU by the exhaustive autotuner, with the 14 selector features.
 
     python -m spmvdare.corpus --count 4096 --seed 33 --out corpus.npz
+
+When variants are added, an existing corpus gains their columns without
+timing the others again: time only the new variants on the same draws, then
+merge (columns in variant-list order):
+
+    python -m spmvdare.corpus --seed 33 --draws-from corpus.npz --variants 40,41 --out new.npz
+    python -m spmvdare.corpus --merge corpus.npz new.npz --out corpus2.npz
 
 Matrices are sampled from the batched workload's parameter ranges (rows = cols
 log-uniform in [1e3, 2e5], avg 4..64, std 0.25..2 x avg, normal / gamma,
@@ -18,17 +25,27 @@ import numpy as np
 from . import gen, hip, tuning
 
 
-def build(count, seed, warmup=2, iters=8, log=None, ranges=None, budget_s=None, start=0, stop=None):
+def build(count, seed, warmup=2, iters=8, log=None, ranges=None, budget_s=None, start=0, stop=None,
+          draws=None, variants=None):
     """Time `count` matrices drawn from sample_params(seed), beginning at draw
     `start`, or (stop given) every matrix of the draws [start, stop), so that
-    a corpus can be built in several sessions and concatenated."""
-    params = tuning.sample_params(seed, stop if stop is not None else start + count * 2, **(ranges or {}))
-    if stop is not None:
-        count = stop - start
+    a corpus can be built in several sessions and concatenated; or (draws
+    given) exactly those draws. `variants`: indices into tuning.VARIANTS to
+    time (default all)."""
+    if draws is not None:
+        draws = [int(d) for d in draws]
+        params = tuning.sample_params(seed, max(draws) + 1 if draws else 0, **(ranges or {}))
+        order = draws
+        count = len(draws)
+    else:
+        params = tuning.sample_params(seed, stop if stop is not None else start + count * 2, **(ranges or {}))
+        if stop is not None:
+            count = stop - start
+        order = range(start, len(params))
     feats, times, lines, nnzs, pidx = [], [], [], [], []
     t0 = time.time()
     k = 0
-    for i in range(start, len(params)):
+    for i in order:
         p = params[i]
         if len(lines) == count:
             break
@@ -39,11 +56,13 @@ def build(count, seed, warmup=2, iters=8, log=None, ranges=None, budget_s=None, 
         try:
             a = gen.generate(p, stats=False)
         except gen.GenError:
+            if draws is not None:
+                raise RuntimeError("draw %d no longer generates" % i)
             continue
         dm = tuning.DeviceCSR(a, p.seed + 1)
         st = dm.stats()
         feats.append(tuning.features(p, st))
-        times.append(tuning.autotune(dm, warmup=warmup, iters=iters))
+        times.append(tuning.autotune(dm, variants=variants, warmup=warmup, iters=iters))
         lines.append(p.line())
         nnzs.append(a.nnz)
         pidx.append(i)
@@ -54,7 +73,30 @@ def build(count, seed, warmup=2, iters=8, log=None, ranges=None, budget_s=None, 
     return np.array(feats), np.array(times), np.array(lines), np.array(nnzs), np.array(pidx)
 
 
+def merge(base, add, out):
+    """Columns of `add` (timed for the variants it names) joined to `base`,
+    row by row on the same draws, in tuning.VARIANTS order."""
+    a, b = np.load(base), np.load(add)
+    if list(a["draw"]) != list(b["draw"]) or list(a["lines"]) != list(b["lines"]):
+        raise SystemExit("merge: the corpora hold different matrices")
+    names = [tuning.variant_name(i) for i in range(len(tuning.VARIANTS))]
+    cols = {n: a["times"][:, j] for j, n in enumerate(a["variants"])}
+    cols.update({n: b["times"][:, j] for j, n in enumerate(b["variants"])})
+    missing = [n for n in names if n not in cols]
+    if missing:
+        raise SystemExit("merge: no timings for %s" % ", ".join(missing))
+    np.savez_compressed(out, features=a["features"], times=np.stack([cols[n] for n in names], axis=1),
+                        lines=a["lines"], nnz=a["nnz"], draw=a["draw"], variants=np.array(names))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--merge":
+        mp = argparse.ArgumentParser()
+        mp.add_argument("--merge", nargs=2, required=True)
+        mp.add_argument("--out", required=True)
+        margs = mp.parse_args()
+        merge(margs.merge[0], margs.merge[1], margs.out)
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=33)
@@ -64,16 +106,20 @@ def main():
     ap.add_argument("--start", type=int, default=0, help="first parameter draw to use")
     ap.add_argument("--stop", type=int, default=None,
                     help="time every matrix of the draws [start, stop) (instead of --count matrices)")
+    ap.add_argument("--draws-from", default=None, help="time the draws of this corpus (same --seed)")
+    ap.add_argument("--variants", default=None, help="comma-separated variant indices (default all)")
     args = ap.parse_args()
+    draws = np.load(args.draws_from)["draw"] if args.draws_from else None
+    vidx = [int(v) for v in args.variants.split(",")] if args.variants else None
     hip.init(0)
 
     def log(s):
         print(s, file=sys.stderr, flush=True)
 
     X, T, L, N, I = build(args.count, args.seed, iters=args.iters, log=log, budget_s=args.budget,
-                          start=args.start, stop=args.stop)
-    np.savez_compressed(args.out, features=X, times=T, lines=L, nnz=N, draw=I,
-                        variants=np.array([tuning.variant_name(i) for i in range(len(tuning.VARIANTS))]))
+                          start=args.start, stop=args.stop, draws=draws, variants=vidx)
+    names = [tuning.variant_name(i) for i in (vidx if vidx is not None else range(len(tuning.VARIANTS)))]
+    np.savez_compressed(args.out, features=X, times=T, lines=L, nnz=N, draw=I, variants=np.array(names))
     log("wrote %s: %d matrices x %d variants" % (args.out, T.shape[0], T.shape[1]))
 
 

@@ -1,9 +1,10 @@
 """Autotuner and learned format selector.
 
 Variant space: 40 format x tuning choices over the batchable drop-in formats
-(row-length-binned CSR with rows-per-wave and lanes per row, SELL with
-lanes per row and its inner unroll, SELL-C-sigma with sigma, lanes per row,
-unroll and x windows). The exhaustive
+(row-length-binned CSR with rows-per-wave, lanes per row, load width and XCD
+chunks; SELL with lanes per row and its inner unroll; SELL-C-sigma with
+sigma, lanes per row, unroll and x windows; DIV-RF with packet size, values
+per decode step and device layout). The exhaustive
 autotuner converts a matrix to every variant on the device and times it; the
 selector is an FP32 MLP 14 -> 64 -> 64 -> 40 (hand-written HIP kernels,
 spmvhip_mlp_*) that predicts every variant's slowdown log(t / t_best) from
@@ -30,24 +31,33 @@ def _variants():
             v.append(("csr_vec", {"block_nnz": bnnz, "lanes_per_row": lanes}))
     for bnnz in (128, 256):
         v.append(("csr_vec", {"block_nnz": bnnz, "lanes_per_row": 4}))
+    # one stored entry per lane and load; row blocks dealt to the XCDs in chunks
+    v.append(("csr_vec", {"block_nnz": 256, "lanes_per_row": 0, "narrow_loads": 1}))
+    v.append(("csr_vec", {"block_nnz": 256, "lanes_per_row": 0, "xcd_remap": 1}))
+    v.append(("csr_vec", {"block_nnz": 512, "lanes_per_row": 0, "xcd_remap": 1}))
+    v.append(("csr_vec", {"block_nnz": 256, "lanes_per_row": 4, "xcd_remap": 1}))
     # SELL-C-sigma, C = 64: sort window sigma x lanes per row T
+    for sigma in (64, 256, 1024, 4096, 16384):
+        v.append(("sell_c_s", {"sigma": sigma, "lanes_per_row": 1, "unroll": 4}))
     for sigma in (64, 256, 1024, 4096):
-        for T in (1, 2, 4, 8):
-            v.append(("sell_c_s", {"sigma": sigma, "lanes_per_row": T, "unroll": 4}))
-    for T in (1, 4):
-        v.append(("sell_c_s", {"sigma": 16384, "lanes_per_row": T, "unroll": 4}))
+        v.append(("sell_c_s", {"sigma": sigma, "lanes_per_row": 2, "unroll": 4}))
+    for sigma in (256, 4096):
+        v.append(("sell_c_s", {"sigma": sigma, "lanes_per_row": 4, "unroll": 4}))
     for unroll in (2, 8):
         v.append(("sell_c_s", {"sigma": 256, "lanes_per_row": 1, "unroll": unroll}))
     v.append(("sell_c_s", {"sigma": 4096, "lanes_per_row": 4, "unroll": 4, "y_mode": 3}))
     # x windows in LDS (banded placements)
-    v.append(("sell_c_s", {"sigma": 256, "lanes_per_row": 4, "unroll": 4, "x_window": 1}))
     v.append(("sell_c_s", {"sigma": 256, "lanes_per_row": 1, "unroll": 4, "x_window": 1}))
     v.append(("sell_c_s", {"sigma": 64, "lanes_per_row": 4, "unroll": 4, "x_window": 4096}))
     # unsorted SELL
-    for T in (1, 2, 4, 8):
+    for T in (1, 2, 4):
         v.append(("sell", {"lanes_per_row": T, "unroll": 4}))
     v.append(("sell", {"lanes_per_row": 1, "unroll": 8}))
-    v.append(("sell", {"lanes_per_row": 4, "unroll": 4, "x_window": 1}))
+    # DIV-RF: packet size P x values per lane and decode step, automatic device layout
+    # (interleaved lane streams), one encoder partition, groups dealt to the XCDs in chunks
+    for P, unroll in ((1024, 8), (1024, 4), (512, 4), (512, 8), (2048, 8)):
+        v.append(("div_rf", {"div_packet_vals": P, "div_threads": 1, "unroll": unroll, "xcd_remap": 1}))
+    v.append(("div_rf", {"div_packet_vals": 1024, "div_threads": 1, "unroll": 8, "xcd_remap": 1, "div_layout": 1}))
     return v
 
 
