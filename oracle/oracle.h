@@ -15,7 +15,10 @@
  *   oracle_sellcs_*       bit-exact against the reference's own conversion
  *                         (oracle/_ref/sellcs_dump, golden fixtures in
  *                         tests/golden/), brute-force stable sort on tiny m.
- *   oracle_sell_*         closed form of the padding rule on hand-built
+ *   oracle_sell_*         bit-exact against the reference's own build
+ *                         (oracle/_ref/sell_dump: cluster sizes, ja with the
+ *                         per-thread padding carry, a and y; golden fixtures),
+ *                         closed form of the padding rule on hand-built
  *                         matrices (empty rows, ragged last slice) at any T.
  *   oracle_sellsorted_*   bit-exact against the reference's own build
  *                         (oracle/_ref/sell_sorted_dump: thread windows,

@@ -7,12 +7,15 @@ reference's generator (artificial-matrix-generator/artificial_matrix_generation.
 and SELL-C-sigma conversion (src/spmv_kernels/sell-C-s/radix_sort.c,
 sellcs_format.c).
 
-    python tests/golden/make_fixtures.py [sell_sorted|div_reorder|div_lossy]   (one set only)
+    python tests/golden/make_fixtures.py [sell|sell_sorted|div_reorder|div_lossy]   (one set only)
 
 Outputs (tests/golden/):
     generator.json  per parameter line: m, n, nnz, FNV-1a 64 of row_ptr /
                     col_ind / raw values, and the reference's descriptor stats
     sellcs.json     per (matrix, C, sigma): FNV-1a 64 of each converted array
+    sell.json       per (matrix, threads): the reference's SELL
+                    (src/spmv_kernels/sell.cpp, 4-row clusters): digests of its
+                    cluster sizes, ja, a and y (x as for sell_sorted.json)
     sell_sorted.json  per (matrix, threads): the reference's SELL_SORTED
                     (src/spmv_kernels/sell_sorted.cpp, AVX2 build: 4-row
                     clusters): thread windows and digests of its permutation,
@@ -178,6 +181,7 @@ def main():
     div_fixtures(gen)
     div_reorder_fixtures(gen)
     div_lossy_fixtures(gen)
+    sell_fixtures(gen)
     sell_sorted_fixtures(gen)
 
 
@@ -294,6 +298,36 @@ def div_reorder_fixtures(gen):
         json.dump(out, f, sort_keys=True)
 
 
+def sell_fixtures(gen):
+    if not os.path.exists(os.path.join(REF, "sell_dump")):
+        sys.exit("oracle/_ref/sell_dump missing: run `make -C oracle ref` first")
+    cases = [("test_txt", 1), ("test_txt", 3), ("test_txt", 8), ("cfg1", 4), ("neigh_two", 2),
+             ("batch_03", 5), ("dense_rows", 1), ("tall", 4), ("stream_01", 7)]
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        for name, T in cases:
+            line = gen[name]["line"]
+            csr = os.path.join(td, "c.bin")
+            run_gen(line, csr)
+            dump = os.path.join(td, "s.bin")
+            subprocess.run([os.path.join(REF, "sell_dump"), csr, dump], check=True, capture_output=True,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(T)))
+            m = gen[name]["m"]
+            with open(dump, "rb") as f:
+                VL, TT, ncl, ne = struct.unpack("<4q", f.read(32))
+                cptr = np.frombuffer(f.read(4 * (ncl + 1)), dtype=np.int32)
+                ja = np.frombuffer(f.read(4 * ne), dtype=np.int32)
+                a = np.frombuffer(f.read(8 * ne), dtype=np.float64)
+                y = np.frombuffer(f.read(8 * m), dtype=np.float64)
+            key = "%s/T%d" % (name, T)
+            out[key] = {"line": line, "VL": int(VL), "T": T, "nnz_ext": int(ne),
+                        "cluster_ptr": digest(cptr.astype(np.int64)), "ja": digest(ja.astype(np.int64)),
+                        "a": digest(a), "y": digest(y)}
+            print(key, int(ne), flush=True)
+    with open(os.path.join(HERE, "sell.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 def sell_sorted_fixtures(gen):
     if not os.path.exists(os.path.join(REF, "sell_sorted_dump")):
         sys.exit("oracle/_ref/sell_sorted_dump missing: run `make -C oracle ref` first")
@@ -328,7 +362,7 @@ def sell_sorted_fixtures(gen):
 
 
 if __name__ == "__main__":
-    if sys.argv[1:] in (["sell_sorted"], ["div_reorder"], ["div_lossy"]):
+    if sys.argv[1:] in (["sell"], ["sell_sorted"], ["div_reorder"], ["div_lossy"]):
         with open(os.path.join(HERE, "generator.json")) as f:
             globals()[sys.argv[1] + "_fixtures"](json.load(f))
     else:

@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 GEN = json.load(open(os.path.join(HERE, "golden", "generator.json")))
 SELL = json.load(open(os.path.join(HERE, "golden", "sellcs.json")))
 SSORT = json.load(open(os.path.join(HERE, "golden", "sell_sorted.json")))
+SELLREF = json.load(open(os.path.join(HERE, "golden", "sell.json")))
 
 
 def digest(a):
@@ -128,6 +129,20 @@ def test_sellsorted_order_brute_force(VL, bounds):
 
 
 # ------------------------------------------------------------ SELL
+
+@pytest.mark.parametrize("key", sorted(SELLREF))
+def test_sell_oracle_matches_reference(key):
+    """Cluster sizes, ja (with the per-thread padding column carry), a and y
+    of the reference's own SELL build (sell.cpp, 4-row clusters, T threads)."""
+    f = SELLREF[key]
+    a = gen.generate(gen.GenParams.parse(f["line"]), stats=False, map_values=False)
+    conv = pyoracle.sell_convert(a.row_ptr, a.col_ind, a.values, f["VL"], T=f["T"])
+    assert int(conv["cluster_ptr"][-1]) == f["nnz_ext"]
+    for k in ("cluster_ptr", "ja", "a"):
+        assert digest(conv[k]) == f[k], k
+    x = ((np.arange(a.n) % 13) + 1) / 8.0 - 1.0
+    assert digest(pyoracle.sell_spmv(conv, f["VL"], x, a.m)) == f["y"]
+
 
 def test_sell_padding_rule_closed_form():
     """sell.cpp:165-192: padding takes value 0 and the column of the previous
