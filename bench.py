@@ -240,120 +240,40 @@ E2E_RANGES = dict(rows=(1e4, 4e6), avg=(2, 128), skew=(1, 1000), distributions=(
 
 
 def run_e2e(args, gen, hip, np):
-    """End-to-end streaming sample of the held-out workload (master seed 55:
-    rows = cols log-uniform in [1e4, 4e6], avg 2..128, gamma rows, skew up to
-    1000; the other descriptors as the batched workload). The first `count`
-    matrices of that stream are processed in chunks that fit HBM: the host
-    generates chunk i + 1 (a worker thread; the generator releases the
-    interpreter lock) while the GPU uploads chunk i, computes the structure
-    statistics, lets the selector choose each matrix's format, converts it and
-    runs the chunk as one batched launch. Compared on the same GPU with plain
-    per-matrix CSR launches; every y is checked against the device csr_naive
-    (bit-identical to the oracle's sequential sum) at 1e-13 normwise; the
-    first `e2e_tuned` matrices are also tuned exhaustively for the regret.
-    Prints one JSON line."""
-    import threading
-    from spmvdare import tuning
+    """End-to-end streaming of the held-out workload (master seed 55: rows =
+    cols log-uniform in [1e4, 4e6], avg 2..128, gamma rows, skew up to 1000;
+    the other descriptors as the batched workload): the first `count`
+    matrices of that stream, in chunks that fit HBM, through the selector (with
+    its fallback to exhaustive tuning) and one batched launch per chunk,
+    against plain per-matrix CSR launches; every y checked against the device
+    csr_naive. spmvdare/stream.py holds the pipeline. Prints one JSON line."""
+    from spmvdare import stream, tuning
     model = os.path.join(ROOT, "spmv-dare_amd", "spmvdare", "models", "selector.json")
     sel = tuning.Selector.load(model)
-    allowed = tuning.batch_allowed()
-    csr = tuning.DEFAULT_CSR
     count = args.e2e_count
-    params = tuning.sample_params(55, count * 2, **E2E_RANGES)
-    chunk = max(1, args.e2e_chunk)
-    stream = iter(params)
-
-    def produce(n):
-        out = []
-        for p in stream:
-            try:
-                out.append((p, gen.generate(p, stats=False)))
-            except gen.GenError:
-                continue
-            if len(out) == n:
-                break
-        return out
-
-    t_start = time.perf_counter()
-    done, totals = 0, {"nnz": 0, "bytes": 0, "ms_batched": 0.0, "ms_plain": 0.0, "gen_s": 0.0}
-    worst, regrets, chosen = 0.0, [], []
-    pending = produce(min(chunk, count))
-    while pending:
-        box = {}
-        want = min(chunk, count - done - len(pending))
-        worker = None
-        if want > 0:
-            worker = threading.Thread(target=lambda: box.update(m=produce(want)))
-            worker.start()
-        mats = []
-        for p, a in pending:
-            dm = tuning.DeviceCSR(a, p.seed + 1)
-            # the selector's top 3, refined by timing them (the fallback to tuning)
-            top = [int(i) for i in sel.select(tuning.features(p, dm.stats()), k=3, allowed=allowed)[0]]
-            tt = [tuning.time_variant(dm, i, 1, 3) for i in top]
-            v = top[int(np.argmin(tt))]
-            chosen.append(tuning.variant_name(v))
-            f_sel, f_csr = dm.convert(tuning.batch_variant(v, args.batch_bnnz)), dm.convert(csr)
-            f_ref = hip.csr_to_format_device(dm.rp, dm.col, dm.val, dm.m, dm.n, dm.nnz, kind="csr_naive")
-            if done + len(mats) < args.e2e_tuned:
-                t = tuning.autotune(dm, warmup=1, iters=5)
-                regrets.append((float(tuning.time_variant(dm, top[0], 1, 5) / t.min() - 1.0),
-                                float(tuning.time_variant(dm, v, 1, 5) / t.min() - 1.0)))
-            mats.append((p, a, dm, f_sel, f_csr, f_ref, hip.DeviceArray(max(a.m, 1))))
-            totals["nnz"] += a.nnz
-            totals["bytes"] += alg_bytes(a.m, a.n, a.nnz)
-        batch, base = hip.Batch(args.tile), hip.Batch(args.tile)
-        batch.set_queues(args.queues)
-        for p, a, dm, f_sel, f_csr, f_ref, dy_csr in mats:
-            batch.add(f_sel, dm.dx, dm.dy)
-            base.add(f_csr, dm.dx, dy_csr)
-        batch.finalize(use_graph=True)
-        base.finalize(use_graph=False)
-        totals["ms_batched"] += float(np.median(batch.time(2, args.e2e_steps)))
-        totals["ms_plain"] += float(np.median(base.time(1, args.e2e_steps, per_matrix=True)))
-        for p, a, dm, f_sel, f_csr, f_ref, dy_csr in mats:
-            yb = dm.dy.download()[:a.m]
-            f_ref.spmv_device(dm.dx, dy_csr)
-            hip.synchronize()
-            yr = dy_csr.download()[:a.m]
-            nrm = np.linalg.norm(yr)
-            err = float(np.linalg.norm(yb - yr) / nrm) if nrm > 0 else float(np.linalg.norm(yb))
-            worst = max(worst, err)
-        done += len(mats)
-        batch.destroy()
-        base.destroy()
-        for m_ in mats:
-            for f in m_[3:6]:
-                f.destroy()
-        del mats
-        log("e2e: %d/%d matrices, %.0f s" % (done, count, time.perf_counter() - t_start))
-        if worker is not None:
-            worker.join()
-        pending = box.get("m", [])
-    t_b, t_p = totals["ms_batched"], totals["ms_plain"]
+    params = tuning.sample_params(55, max(count * 2, count + 64), **E2E_RANGES)
+    r = stream.run_stream(params, count, max(1, args.e2e_chunk), sel, tuned=args.e2e_tuned,
+                          fallback_margin=args.fallback_margin, steps=args.e2e_steps, batch_bnnz=args.batch_bnnz,
+                          queues=args.queues, tile=args.tile, log=log)
     out = {
-        "metric": "end-to-end batched FP64 SpMV matrices/s (selector-chosen formats), streaming sample",
-        "value": round(done / (t_b * 1e-3), 1), "unit": "matrices/s", "n_gpus": 1, "higher_is_better": True,
+        "metric": "end-to-end batched FP64 SpMV matrices/s (selector-chosen formats), streaming",
+        "value": round(r["matrices_per_s"], 1), "unit": "matrices/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f64", "data": "synthetic (reference generator restated bit-exactly)",
         "config": {"workload": "first %d matrices of the master-seed-55 stream (rows 1e4..4e6, avg 2..128, gamma, "
-                               "skew <= 1000), chunks of %d" % (done, chunk),
-                   "nnz": totals["nnz"], "alg_bytes": totals["bytes"]},
-        "batched_ms": round(t_b, 3), "batched_gbps": round(totals["bytes"] / (t_b * 1e-3) / 1e9, 1),
-        "plain_per_matrix_csr": {"ms": round(t_p, 3), "matrices_per_s": round(done / (t_p * 1e-3), 1),
-                                 "gbps": round(totals["bytes"] / (t_p * 1e-3) / 1e9, 1)},
-        "speedup_vs_plain_csr": round(t_p / t_b, 3),
-        "batch_min_block_nnz": args.batch_bnnz,
-        "max_normwise_err_vs_csr_naive": worst,
-        "selector_regret_vs_exhaustive": {
-            "matrices": len(regrets),
-            "top1_mean": float(np.mean([r[0] for r in regrets])) if regrets else None,
-            "top3_refined_mean": float(np.mean([r[1] for r in regrets])) if regrets else None,
-            "top3_refined_max": float(np.max([r[1] for r in regrets])) if regrets else None},
-        "wall_s": round(time.perf_counter() - t_start, 1),
-        "chosen_variants": {k: chosen.count(k) for k in sorted(set(chosen))},
+                               "skew <= 1000), chunks of %d" % (r["matrices"], r["chunk"]),
+                   "nnz": r["nnz"], "alg_bytes": r["alg_bytes"]},
+        "batched_ms": round(r["batched_ms"], 3), "batched_gbps": round(r["batched_gbps"], 1),
+        "plain_per_matrix_csr": {"ms": round(r["plain_per_matrix_csr_ms"], 3),
+                                 "matrices_per_s": round(r["plain_matrices_per_s"], 1)},
+        "speedup_vs_plain_csr": round(r["speedup_vs_plain_csr"], 3),
+        "max_normwise_err_vs_csr_naive": r["max_normwise_err_vs_csr_naive"],
+        "selector": {"fallback_margin": r["fallback_margin"], "fallbacks_to_exhaustive": r["fallbacks"],
+                     "regret_vs_exhaustive": r["regret"]},
+        "wall_s": round(r["wall_s"], 1), "chosen_variants": r["chosen_variants"],
     }
-    log("e2e %d matrices: batched %.2f ms vs per-matrix CSR %.2f ms (x%.2f), max err %.2e" % (
-        done, t_b, t_p, t_p / t_b, worst))
+    log("e2e %d matrices: batched %.2f ms vs per-matrix CSR %.2f ms (x%.2f), max err %.2e, %d fallbacks" % (
+        r["matrices"], r["batched_ms"], r["plain_per_matrix_csr_ms"], r["speedup_vs_plain_csr"],
+        r["max_normwise_err_vs_csr_naive"], r["fallbacks"]))
     print(json.dumps(out), flush=True)
 
 
@@ -484,9 +404,11 @@ def main():
     ap.add_argument("--queues", type=int, default=256,
                     help="batched launch: ticket counters the units are dealt over (a power of two)")
     ap.add_argument("--e2e", action="store_true", help="end-to-end streaming sample of the held-out workload")
-    ap.add_argument("--e2e-count", type=int, default=32, help="matrices of the streaming sample")
-    ap.add_argument("--e2e-chunk", type=int, default=8, help="matrices per chunk of the streaming sample")
-    ap.add_argument("--e2e-tuned", type=int, default=4, help="matrices of the sample also tuned exhaustively")
+    ap.add_argument("--e2e-count", type=int, default=1000, help="matrices of the stream")
+    ap.add_argument("--e2e-chunk", type=int, default=16, help="matrices per chunk of the stream")
+    ap.add_argument("--e2e-tuned", type=int, default=32, help="matrices of the stream also tuned exhaustively")
+    ap.add_argument("--fallback-margin", type=float, default=0.05,
+                    help="selector falls back to exhaustive tuning when its two best predictions are this close")
     ap.add_argument("--e2e-steps", type=int, default=10, help="timed launches per chunk")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=128)

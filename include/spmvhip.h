@@ -233,6 +233,12 @@ int spmvhip_batch_set_grid(spmvhip_batch *b, int workgroups_per_cu);
  * XCDs' L2; 0 = every wave draws from the whole list; -1 (default) =
  * automatic: ranges when no member holds more than 1/64 of the units. */
 int spmvhip_batch_set_xcd_ranges(spmvhip_batch *b, int on);
+/* Large members (before any add): a member of at least nnz nonzeros (0, the
+ * default: none) runs as its own kernels -- the ones spmvhip_spmv_device
+ * runs -- at the start of the same launch sequence (and hipGraph), ahead of
+ * the persistent kernels, instead of as units in them: such a matrix fills
+ * the GPU alone, and its tuned standalone kernels beat the shared unit loop. */
+int spmvhip_batch_set_solo_nnz(spmvhip_batch *b, long nnz);
 int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d_x, double *d_y);
 int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph);
 int spmvhip_batch_run(spmvhip_batch *b);          /* asynchronous on the batch's stream */
@@ -285,6 +291,11 @@ int spmvhip_mlp_gradients(spmvhip_mlp *net, const float *X, const float *T, cons
 int spmvhip_mlp_train(spmvhip_mlp *net, const float *X, const float *T, const float *M, long rows,
 		int batch, int epochs, float lr, float *epoch_losses);
 int spmvhip_mlp_adam_step(spmvhip_mlp *net, const float *grads, float lr);
+/* Adam state for checkpoint / resume: the first and second moment estimates
+ * (count = spmvhip_mlp_num_params floats each) and the step counter.
+ * set_params resets it; set_adam_state after set_params restores it. */
+int spmvhip_mlp_get_adam_state(const spmvhip_mlp *net, float *m1, float *m2, long count, long *step);
+int spmvhip_mlp_set_adam_state(spmvhip_mlp *net, const float *m1, const float *m2, long count, long step);
 void spmvhip_mlp_destroy(spmvhip_mlp *net);
 
 /* The host half of csr_to_format for DIV_RF, exposed so that the encoder can

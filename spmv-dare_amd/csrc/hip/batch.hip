@@ -250,6 +250,15 @@ struct spmvhip_batch {
 	long total_nnz = 0;
 	std::vector<int> div_fmts;           // indices into fmts of the DIV members
 	std::vector<std::pair<uint64_t, uint64_t>> div_xy; // their x and y
+	// members of at least solo_nnz nonzeros (0: none) run as their own
+	// kernels, ahead of the persistent kernels, in the same launch (and graph)
+	long solo_nnz = 0;
+	struct Solo {
+		spmvhip_format *f;
+		const double *x;
+		double *y;
+	};
+	std::vector<Solo> solo;
 };
 
 using namespace spmvhip;
@@ -258,6 +267,9 @@ namespace {
 
 int batch_launch(spmvhip_batch *b, hipStream_t s)
 {
+	// large members first, each filling the GPU with its own kernels
+	for (const auto &m : b->solo)
+		SPMVHIP_TRY(format_spmv(m.f, m.x, m.y, s));
 	batch_reset<<<1, kCountersMax, 0, s>>>(b->counter.as<int>(), b->nc * b->nranges);
 	SPMVHIP_CHECK(hipGetLastError());
 	if (b->nunits > 0) {
@@ -342,12 +354,32 @@ int spmvhip_batch_set_grid(spmvhip_batch *b, int workgroups_per_cu)
 	return SPMVHIP_OK;
 }
 
+int spmvhip_batch_set_solo_nnz(spmvhip_batch *b, long nnz)
+{
+	if (!b)
+		return set_error(SPMVHIP_EINVAL, "NULL batch");
+	if (b->finalized)
+		return set_error(SPMVHIP_EINVAL, "batch already finalized");
+	if (!b->fmts.empty())
+		return set_error(SPMVHIP_EINVAL, "batch: set the solo threshold before adding members");
+	if (nnz < 0)
+		return set_error(SPMVHIP_EINVAL, "batch: solo threshold %ld < 0", nnz);
+	b->solo_nnz = nnz;
+	return SPMVHIP_OK;
+}
+
 int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d_x, double *d_y)
 {
 	if (!b || !f || !d_x || !d_y)
 		return set_error(SPMVHIP_EINVAL, "NULL argument");
 	if (b->finalized)
 		return set_error(SPMVHIP_EINVAL, "batch already finalized");
+	if (b->solo_nnz > 0 && f->nnz >= b->solo_nnz && f->kind != SPMVHIP_CSR_NAIVE) {
+		b->solo.push_back({const_cast<spmvhip_format *>(f), d_x, d_y});
+		b->fmts.push_back(const_cast<spmvhip_format *>(f));
+		b->total_nnz += f->nnz;
+		return SPMVHIP_OK;
+	}
 	BatchMat M{};
 	M.m = f->m;
 	M.x = ptr_bits(d_x);

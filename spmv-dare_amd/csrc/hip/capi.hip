@@ -3,6 +3,7 @@
 
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -45,6 +46,28 @@ static int ensure_device()
 		return set_error(SPMVHIP_ENODEV, "no HIP device available (%s)",
 				e != hipSuccess ? hipGetErrorString(e) : "0 devices");
 	ready = 1;
+	return SPMVHIP_OK;
+}
+
+// One non-blocking stream per device, shared by every format handle on that
+// device (its conversion, and the spmv calls that name no stream). The HIP
+// runtime backs each stream with host resources of its own (a worker thread
+// per queue), so a stream per handle does not scale to the thousands of
+// handles a batched workload holds (two or three per matrix).
+constexpr int kMaxDevices = 64;
+static std::mutex g_stream_mu;
+static hipStream_t g_streams[kMaxDevices];
+
+static int shared_stream(hipStream_t *out)
+{
+	int dev = 0;
+	SPMVHIP_CHECK(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kMaxDevices)
+		return set_error(SPMVHIP_EINVAL, "device %d beyond %d", dev, kMaxDevices);
+	std::lock_guard<std::mutex> lock(g_stream_mu);
+	if (!g_streams[dev])
+		SPMVHIP_CHECK(hipStreamCreateWithFlags(&g_streams[dev], hipStreamNonBlocking));
+	*out = g_streams[dev];
 	return SPMVHIP_OK;
 }
 
@@ -108,9 +131,12 @@ static int create_handle(long m, long n, long nnz, const spmvhip_params *params,
 	f->nnz = nnz;
 	f->name = kind_name(f->kind);
 	f->csr_mem_footprint = (double) nnz * (sizeof(double) + sizeof(int32_t)) + (double) (m + 1) * sizeof(int32_t);
-	hipError_t e = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking);
-	if (e == hipSuccess)
-		e = hipEventCreate(&f->ev0);
+	int src = shared_stream(&f->stream);
+	if (src) {
+		delete f;
+		return src;
+	}
+	hipError_t e = hipEventCreate(&f->ev0);
 	if (e == hipSuccess)
 		e = hipEventCreate(&f->ev1);
 	if (e != hipSuccess) {
@@ -119,6 +145,13 @@ static int create_handle(long m, long n, long nnz, const spmvhip_params *params,
 	}
 	*out = f;
 	return SPMVHIP_OK;
+}
+
+static int run_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s);
+
+int format_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
+{
+	return run_spmv(f, x, y, s);
 }
 
 static int run_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
@@ -534,10 +567,7 @@ void spmvhip_destroy(spmvhip_format *f)
 		(void) hipEventDestroy(f->ev0);
 	if (f->ev1)
 		(void) hipEventDestroy(f->ev1);
-	hipStream_t s = f->stream;
-	delete f; // DevBuf members free their memory
-	if (s)
-		(void) hipStreamDestroy(s);
+	delete f; // DevBuf members free their memory; the stream is the device's shared one
 }
 
 } // extern "C"

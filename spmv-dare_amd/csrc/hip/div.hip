@@ -155,9 +155,11 @@ __device__ __forceinline__ uint64_t load_le_fast(const uint8_t *__restrict__ bas
 // together (coalesced across lanes), the K stream offsets follow from a
 // running sum of the lengths in registers, then the K deltas and their
 // coordinates are requested together and the value prefix sum runs in
-// registers. Products go to the packet's rows in LDS (atomic adds; the
-// order of the additions is not fixed), which are added to y at the end of
-// the packet.
+// registers. Products go to the packet's rows in LDS (atomic adds: the
+// lanes of one instruction that hit the same row are applied by the LDS in a
+// fixed hardware order, not the sequential order of csr_naive, so y agrees
+// with csr_naive to rounding and repeated launches give the same bits),
+// which are added to y at the end of the packet.
 template <int K>
 __global__ void __launch_bounds__(kBlock) div_spmv_kernel(const uint8_t *__restrict__ bytes,
 		const int64_t *__restrict__ pk_off, const int32_t *__restrict__ grp_pk, const int32_t *__restrict__ grp_row,

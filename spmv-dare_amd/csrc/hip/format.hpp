@@ -10,7 +10,7 @@ struct spmvhip_format {
 	int kind = 0;
 	long m = 0, n = 0, nnz = 0;
 	const char *name = "";
-	hipStream_t stream = nullptr;
+	hipStream_t stream = nullptr; // the device's stream shared by all handles (capi.hip)
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
 	// CSR arrays (all CSR kinds; DIV keeps them only until encoded)
@@ -91,6 +91,8 @@ int div_convert(spmvhip_format *f, const int32_t *h_row_ptr, const int32_t *h_co
 		const double *h_val);
 int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s);
 int div_export_decoded(const spmvhip_format *f, int which, void *buf, size_t *bytes);
+// The format's own SpMV on stream s (the kernels spmvhip_spmv_device runs).
+int format_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s);
 
 // DIV-RF members of a batched launch (batch.hip): one persistent kernel over
 // the packet groups of all of them (div.hip).

@@ -266,6 +266,29 @@ int spmvhip_mlp_get_params(const spmvhip_mlp *n, float *host, long count)
 	return SPMVHIP_OK;
 }
 
+int spmvhip_mlp_get_adam_state(const spmvhip_mlp *n, float *m1, float *m2, long count, long *step)
+{
+	if (!n || !m1 || !m2 || !step || count != n->nparams)
+		return set_error(SPMVHIP_EINVAL, "mlp_get_adam_state: expected %ld floats per moment", n ? n->nparams : 0L);
+	SPMVHIP_CHECK(hipStreamSynchronize(n->stream));
+	SPMVHIP_CHECK(hipMemcpy(m1, n->m1.p, count * sizeof(float), hipMemcpyDeviceToHost));
+	SPMVHIP_CHECK(hipMemcpy(m2, n->m2.p, count * sizeof(float), hipMemcpyDeviceToHost));
+	*step = n->step;
+	return SPMVHIP_OK;
+}
+
+int spmvhip_mlp_set_adam_state(spmvhip_mlp *n, const float *m1, const float *m2, long count, long step)
+{
+	if (!n || !m1 || !m2 || count != n->nparams || step < 0)
+		return set_error(SPMVHIP_EINVAL, "mlp_set_adam_state: expected %ld floats per moment and step >= 0",
+				n ? n->nparams : 0L);
+	SPMVHIP_CHECK(hipStreamSynchronize(n->stream));
+	SPMVHIP_CHECK(hipMemcpy(n->m1.p, m1, count * sizeof(float), hipMemcpyHostToDevice));
+	SPMVHIP_CHECK(hipMemcpy(n->m2.p, m2, count * sizeof(float), hipMemcpyHostToDevice));
+	n->step = step;
+	return SPMVHIP_OK;
+}
+
 // Host in, host out: Y[B][n_out] = mlp(X[B][n_in]).
 int spmvhip_mlp_predict(spmvhip_mlp *n, const float *X, int B, float *Y)
 {

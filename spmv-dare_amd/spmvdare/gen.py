@@ -8,6 +8,7 @@ src/bench.cpp:562-585):
     bw skew avg_num_neighbours cross_row_similarity seed
 """
 import ctypes
+import hashlib
 import os
 from dataclasses import dataclass, fields
 
@@ -130,6 +131,34 @@ def generate(p, stats=True, map_values=True):
     if map_values:
         map_values_inplace(va)
     return CSRMatrix(m, c.n, rp, ci, va, st)
+
+
+def generate_cached(p, cache_dir, stats=False, map_values=True):
+    """generate(p) through an on-disk cache keyed by the 11 parameters (and
+    the value mapping): the first call stores row_ptr / col_ind / values as
+    .npy files under cache_dir, later calls memory-map them. Without stats
+    (the cache does not keep them)."""
+    if stats:
+        raise ValueError("generate_cached keeps no statistics; use generate(p, stats=True)")
+    key = hashlib.sha256(("%s|%d" % (p.line(), 1 if map_values else 0)).encode()).hexdigest()[:32]
+    d = os.path.join(cache_dir, key)
+    names = ("row_ptr.npy", "col_ind.npy", "values.npy", "shape.npy")
+    if all(os.path.exists(os.path.join(d, f)) for f in names):
+        m, n = (int(v) for v in np.load(os.path.join(d, "shape.npy")))
+        rp, ci, va = (np.load(os.path.join(d, f), mmap_mode="r") for f in names[:3])
+        return CSRMatrix(m, n, rp, ci, va, {})
+    a = generate(p, stats=False, map_values=map_values)
+    tmp = d + ".tmp%d" % os.getpid()
+    os.makedirs(tmp, exist_ok=True)
+    for f, arr in zip(names, (a.row_ptr, a.col_ind, a.values, np.array([a.m, a.n], dtype=np.int64))):
+        np.save(os.path.join(tmp, f), arr)
+    try:
+        os.rename(tmp, d)
+    except OSError:  # another process stored it first
+        for f in names:
+            os.remove(os.path.join(tmp, f))
+        os.rmdir(tmp)
+    return a
 
 
 def map_values_inplace(values):

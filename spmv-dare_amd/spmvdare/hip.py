@@ -75,6 +75,7 @@ _sig = {
     "spmvhip_batch_set_queues": ([_vp, ctypes.c_int], ctypes.c_int),
     "spmvhip_batch_set_grid": ([_vp, ctypes.c_int], ctypes.c_int),
     "spmvhip_batch_set_xcd_ranges": ([_vp, ctypes.c_int], ctypes.c_int),
+    "spmvhip_batch_set_solo_nnz": ([_vp, ctypes.c_long], ctypes.c_int),
     "spmvhip_batch_add": ([_vp, _vp, _vp, _vp], ctypes.c_int),
     "spmvhip_batch_finalize": ([_vp, ctypes.c_int], ctypes.c_int),
     "spmvhip_batch_run": ([_vp], ctypes.c_int),
@@ -96,6 +97,8 @@ _sig = {
     "spmvhip_mlp_train": ([_vp, _vp, _vp, _vp, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                            _vp], ctypes.c_int),
     "spmvhip_mlp_adam_step": ([_vp, _vp, ctypes.c_float], ctypes.c_int),
+    "spmvhip_mlp_get_adam_state": ([_vp, _vp, _vp, ctypes.c_long, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+    "spmvhip_mlp_set_adam_state": ([_vp, _vp, _vp, ctypes.c_long, ctypes.c_long], ctypes.c_int),
     "spmvhip_mlp_destroy": ([_vp], None),
     "spmvhip_div_encode": ([_vp, _vp, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_int,
                             ctypes.c_int, ctypes.c_long, _vp, ctypes.POINTER(ctypes.c_size_t), _vp,
@@ -303,6 +306,11 @@ class Batch:
         """Resident workgroups per CU of the persistent launch (0 = default)."""
         _check(_lib.spmvhip_batch_set_grid(self._h, workgroups_per_cu), "batch_set_grid")
 
+    def set_solo_nnz(self, nnz):
+        """Members of at least nnz nonzeros run as their own kernels in the
+        same launch (0: none). Before any add."""
+        _check(_lib.spmvhip_batch_set_solo_nnz(self._h, int(nnz)), "batch_set_solo_nnz")
+
     def set_xcd_ranges(self, on):
         """One contiguous, nnz-balanced range of units per XCD: True / False,
         or -1 for the automatic choice (the default)."""
@@ -419,6 +427,20 @@ class MLP:
     def adam_step(self, grads, lr):
         g = np.ascontiguousarray(grads, dtype=np.float32)
         _check(_lib.spmvhip_mlp_adam_step(self._h, g.ctypes.data, lr), "mlp_adam_step")
+
+    def get_adam_state(self):
+        """(m1, m2, step): Adam's moment estimates and step counter."""
+        n = self.nparams
+        m1, m2, step = np.empty(n, np.float32), np.empty(n, np.float32), ctypes.c_long()
+        _check(_lib.spmvhip_mlp_get_adam_state(self._h, m1.ctypes.data, m2.ctypes.data, n, ctypes.byref(step)),
+               "mlp_get_adam_state")
+        return m1, m2, step.value
+
+    def set_adam_state(self, m1, m2, step):
+        a = np.ascontiguousarray(m1, dtype=np.float32)
+        b = np.ascontiguousarray(m2, dtype=np.float32)
+        _check(_lib.spmvhip_mlp_set_adam_state(self._h, a.ctypes.data, b.ctypes.data, a.size, int(step)),
+               "mlp_set_adam_state")
 
     def destroy(self):
         if self._h:

@@ -3,8 +3,8 @@
 Variant space: 40 format x tuning choices over the batchable drop-in formats
 (row-length-binned CSR with rows-per-wave, lanes per row, load width and XCD
 chunks; SELL with lanes per row and its inner unroll; SELL-C-sigma with
-sigma, lanes per row, unroll and x windows; DIV-RF with packet size, values
-per decode step and device layout). The exhaustive
+sigma, lanes per row, unroll and x windows; SELL_SORTED; DIV-RF with packet
+size, values per decode step, device layout and packet reordering). The exhaustive
 autotuner converts a matrix to every variant on the device and times it; the
 selector is an FP32 MLP 14 -> 64 -> 64 -> 40 (hand-written HIP kernels,
 spmvhip_mlp_*) that predicts every variant's slowdown log(t / t_best) from
@@ -14,8 +14,10 @@ runs; its regret is judged against exhaustive tuning, and with `refine` the
 top-k predicted variants are timed and the fastest kept (the fallback).
 Nothing here has a counterpart in the reference.
 """
+import hashlib
 import json
 import math
+import os
 import random
 
 import numpy as np
@@ -39,10 +41,9 @@ def _variants():
     # SELL-C-sigma, C = 64: sort window sigma x lanes per row T
     for sigma in (64, 256, 1024, 4096, 16384):
         v.append(("sell_c_s", {"sigma": sigma, "lanes_per_row": 1, "unroll": 4}))
-    for sigma in (64, 256, 1024, 4096):
+    for sigma in (64, 256, 4096):
         v.append(("sell_c_s", {"sigma": sigma, "lanes_per_row": 2, "unroll": 4}))
-    for sigma in (256, 4096):
-        v.append(("sell_c_s", {"sigma": sigma, "lanes_per_row": 4, "unroll": 4}))
+    v.append(("sell_c_s", {"sigma": 256, "lanes_per_row": 4, "unroll": 4}))
     for unroll in (2, 8):
         v.append(("sell_c_s", {"sigma": 256, "lanes_per_row": 1, "unroll": unroll}))
     v.append(("sell_c_s", {"sigma": 4096, "lanes_per_row": 4, "unroll": 4, "y_mode": 3}))
@@ -50,7 +51,7 @@ def _variants():
     v.append(("sell_c_s", {"sigma": 256, "lanes_per_row": 1, "unroll": 4, "x_window": 1}))
     v.append(("sell_c_s", {"sigma": 64, "lanes_per_row": 4, "unroll": 4, "x_window": 4096}))
     # unsorted SELL
-    for T in (1, 2, 4):
+    for T in (1, 2):
         v.append(("sell", {"lanes_per_row": T, "unroll": 4}))
     v.append(("sell", {"lanes_per_row": 1, "unroll": 8}))
     # DIV-RF: packet size P x values per lane and decode step, automatic device layout
@@ -58,6 +59,10 @@ def _variants():
     for P, unroll in ((1024, 8), (1024, 4), (512, 4), (512, 8), (2048, 8)):
         v.append(("div_rf", {"div_packet_vals": P, "div_threads": 1, "unroll": unroll, "xcd_remap": 1}))
     v.append(("div_rf", {"div_packet_vals": 1024, "div_threads": 1, "unroll": 8, "xcd_remap": 1, "div_layout": 1}))
+    # SELL_SORTED (ascending sort in row windows) and DIV-RF with the packet groups in median-column order
+    v.append(("sell_sorted", {"sigma": 4096, "lanes_per_row": 1, "unroll": 4}))
+    v.append(("sell_sorted", {"sigma": 256, "lanes_per_row": 2, "unroll": 4}))
+    v.append(("div_rf", {"div_packet_vals": 1024, "div_threads": 1, "unroll": 8, "xcd_remap": 1, "div_reorder": 1}))
     return v
 
 
@@ -174,10 +179,59 @@ def time_variant(dm, i, warmup=2, iters=8):
     return float(np.median(ms))
 
 
-def autotune(dm, variants=None, warmup=2, iters=8):
-    """Exhaustive: median time (ms) of every variant."""
-    idx = range(len(VARIANTS)) if variants is None else variants
-    return np.array([time_variant(dm, i, warmup, iters) for i in idx])
+def autotune(dm, variants=None, warmup=2, iters=8, cache=None, key=None):
+    """Exhaustive: median time (ms) of every variant. With an AutotuneCache
+    and the matrix's key, variants timed before on this device are read from
+    the cache and the new timings are added to it."""
+    idx = list(range(len(VARIANTS)) if variants is None else variants)
+    known = cache.get(key) if cache is not None and key is not None else {}
+    out = []
+    for i in idx:
+        name = variant_name(i)
+        if name not in known:
+            known[name] = time_variant(dm, i, warmup, iters)
+        out.append(known[name])
+    if cache is not None and key is not None:
+        cache.put(key, known)
+    return np.array(out)
+
+
+class AutotuneCache:
+    """Autotuner timings kept on disk (JSON): per device name and matrix key
+    (matrix_key), the median ms of every variant timed so far. Written back
+    on every put, through a temporary file, so an interrupted tuning run
+    resumes where it stopped."""
+
+    def __init__(self, path, device=None):
+        self.path = path
+        self.device = device or hip.device_name()
+        try:
+            with open(path) as f:
+                self.data = json.load(f)
+        except (OSError, ValueError):
+            self.data = {}
+
+    def get(self, key):
+        return dict(self.data.get(self.device, {}).get(key, {}))
+
+    def put(self, key, timings):
+        self.data.setdefault(self.device, {})[key] = dict(timings)
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(self.data, f)
+        os.replace(tmp, self.path)
+
+
+def matrix_key(a, params=None):
+    """Cache key of a matrix: its generator parameter line when it was
+    generated (the 11 parameters fix it), else a digest of its arrays."""
+    if params is not None:
+        return "gen:" + params.line()
+    h = hashlib.sha256()
+    h.update(np.array([a.m, a.n, a.nnz], dtype=np.int64).tobytes())
+    for arr in (a.row_ptr, a.col_ind, a.values):
+        h.update(np.ascontiguousarray(arr).tobytes())
+    return "sha256:" + h.hexdigest()
 
 
 # ------------------------------------------------------------------ selector
@@ -200,12 +254,16 @@ class Selector:
         t = np.asarray(times, dtype=np.float64)
         return np.log(t / t.min(axis=1, keepdims=True))
 
-    def fit(self, X, times, epochs=300, batch=128, lr=2e-3, shuffle_seed=0):
+    def fit(self, X, times, epochs=300, batch=128, lr=2e-3, shuffle_seed=0, resume=False):
+        """Train on (features, timings). resume: continue from the current
+        weights, feature scaling and Adam state (a loaded checkpoint) instead
+        of a fresh network."""
         X = np.asarray(X, dtype=np.float64)
-        self.mu = X.mean(axis=0)
-        self.sd = X.std(axis=0) + 1e-9
         T = self.targets(times)
-        self.net = hip.MLP(X.shape[1], self.hidden[0], self.hidden[1], T.shape[1], seed=self.seed)
+        if not (resume and self.net is not None):
+            self.mu = X.mean(axis=0)
+            self.sd = X.std(axis=0) + 1e-9
+            self.net = hip.MLP(X.shape[1], self.hidden[0], self.hidden[1], T.shape[1], seed=self.seed)
         rng = np.random.default_rng(shuffle_seed)
         Xn = self._norm(X).astype(np.float32)
         losses = []
@@ -226,10 +284,17 @@ class Selector:
             pred = np.where(np.asarray(allowed, dtype=bool)[None, :], pred, np.inf)
         return np.argsort(pred, axis=1)[:, :k]
 
-    def save(self, path):
-        json.dump({"hidden": list(self.hidden), "mu": self.mu.tolist(), "sd": self.sd.tolist(),
-                   "dims": list(self.net.dims), "params": self.net.get_params().tolist(),
-                   "variants": [variant_name(i) for i in range(len(VARIANTS))]}, open(path, "w"))
+    def save(self, path, adam=False):
+        """Weights, feature scaling and variant list as JSON; with adam, also
+        Adam's moments and step (a checkpoint that fit(resume=True) continues)."""
+        d = {"hidden": list(self.hidden), "mu": self.mu.tolist(), "sd": self.sd.tolist(),
+             "dims": list(self.net.dims), "params": self.net.get_params().tolist(),
+             "variants": [variant_name(i) for i in range(len(VARIANTS))]}
+        if adam:
+            m1, m2, step = self.net.get_adam_state()
+            d["adam"] = {"m1": m1.tolist(), "m2": m2.tolist(), "step": int(step)}
+        with open(path, "w") as f:
+            json.dump(d, f)
 
     @classmethod
     def load(cls, path):
@@ -245,6 +310,9 @@ class Selector:
         s.sd = np.array(d["sd"])
         s.net = hip.MLP(*d["dims"])
         s.net.set_params(np.array(d["params"], dtype=np.float32))
+        if "adam" in d:
+            s.net.set_adam_state(np.array(d["adam"]["m1"], dtype=np.float32),
+                                 np.array(d["adam"]["m2"], dtype=np.float32), d["adam"]["step"])
         return s
 
 

@@ -135,3 +135,21 @@ def test_degrees_match_generate():
     d, nnz = gen.degrees(p)
     a = gen.generate(p, stats=False)
     assert nnz == a.nnz and (np.diff(a.row_ptr) == d).all()
+
+
+def test_generate_cached_round_trip(tmp_path):
+    """The on-disk matrix cache returns the generated arrays unchanged, from
+    disk on the second call, keyed by the parameters and the value mapping."""
+    p = gen.GenParams.parse(FIX["test_txt"]["line"])
+    a = gen.generate(p, stats=False)
+    c1 = gen.generate_cached(p, str(tmp_path))
+    c2 = gen.generate_cached(p, str(tmp_path))
+    assert isinstance(c2.values, np.memmap)
+    for c in (c1, c2):
+        assert (c.m, c.n) == (a.m, a.n)
+        np.testing.assert_array_equal(c.row_ptr, a.row_ptr)
+        np.testing.assert_array_equal(c.col_ind, a.col_ind)
+        np.testing.assert_array_equal(c.values, a.values)
+    raw = gen.generate_cached(p, str(tmp_path), map_values=False)
+    np.testing.assert_array_equal(raw.values, gen.generate(p, stats=False, map_values=False).values)
+    assert len(os.listdir(tmp_path)) == 2

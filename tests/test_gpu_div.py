@@ -197,3 +197,31 @@ def test_div_lossy_values(name, tol, layout):
     y_exact = pyoracle.csr_naive(rp, ci, va, x)
     y_abs = pyoracle.csr_naive(rp, ci, np.abs(va), np.abs(x))
     assert np.linalg.norm(y - y_exact) <= tol * np.linalg.norm(y_abs) * 1.001 + 1e-300
+
+
+@pytest.mark.parametrize("name", ["cfg1", "dense_rows", "huge_rows", "batch_03"])
+@pytest.mark.parametrize("layout,copies", [(1, 1), (1, 0), (3, 0), (3, 4), (2, 2)])
+def test_div_repeated_launches_bit_identical(name, layout, copies):
+    """The products reach their row accumulator in LDS through atomic adds
+    (ds_add_f64); lanes of one instruction that hit the same accumulator are
+    applied by the LDS in a fixed lane order, instructions in program order,
+    and the accumulator copies are summed in a fixed order, so repeated
+    launches give the same bits. Checked over 8 launches per case."""
+    rp, ci, va, n = wide_or_matrix(name)
+    m = rp.size - 1
+    print("case repeat", name, layout, copies, flush=True)
+    x = gen.make_x(n, 87)
+    f = hip.csr_to_format(rp, ci, va, m, n, ci.size, kind="div_rf", div_layout=layout, div_acc_copies=copies,
+                          div_packet_vals=512 if layout == 2 else 4096)
+    dx = hip.DeviceArray.from_host(x)
+    dy = hip.DeviceArray(max(m, 1))
+    first = None
+    for _ in range(8):
+        f.spmv_device(dx, dy)
+        hip.synchronize()
+        y = dy.download()[:m]
+        if first is None:
+            first = y
+        else:
+            assert np.array_equal(y.view(np.uint64), first.view(np.uint64))
+    check_dd(first, rp, ci, va, x, "div_rf/repeat/layout%d/copies%d/%s" % (layout, copies, name))

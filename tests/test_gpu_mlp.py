@@ -76,3 +76,23 @@ def test_training_fits_a_smooth_target():
     p = net.get_params().astype(np.float64)
     _, acts = mlp_ref.forward(p, DIMS, X[:32].astype(np.float64))
     np.testing.assert_allclose(net.predict(X[:32]), acts[-1], rtol=1e-3, atol=1e-3)
+
+
+def test_checkpoint_resume_continues_training_exactly():
+    """Weights + Adam moments + step saved mid-training and restored into a
+    fresh network continue exactly as the uninterrupted run (same kernels,
+    same order, so the same bits)."""
+    rng = np.random.default_rng(13)
+    X = rng.normal(0, 1, (512, DIMS[0])).astype(np.float32)
+    T = np.tanh(X @ rng.normal(0, 0.3, (DIMS[0], DIMS[3]))).astype(np.float32)
+    a = hip.MLP(*DIMS, seed=5)
+    a.train(X, T, batch=64, epochs=3, lr=2e-3)
+    params, (m1, m2, step) = a.get_params(), a.get_adam_state()
+    assert step > 0 and np.abs(m2).sum() > 0
+    a.train(X, T, batch=64, epochs=2, lr=2e-3)
+    b = hip.MLP(*DIMS, seed=99)
+    b.set_params(params)
+    b.set_adam_state(m1, m2, step)
+    b.train(X, T, batch=64, epochs=2, lr=2e-3)
+    np.testing.assert_array_equal(b.get_params(), a.get_params())
+    assert b.get_adam_state()[2] == a.get_adam_state()[2]

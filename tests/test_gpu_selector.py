@@ -72,3 +72,19 @@ def test_committed_selector_regret():
     print(json.dumps(res, indent=1))
     assert res["mlp_top1"]["mean"] < res["static_best_variant"]["mean"]
     assert res["mlp_top3_refined"]["mean"] <= 0.05
+
+
+def test_autotune_cache_reuses_timings(tmp_path):
+    """Timings of a matrix already tuned on this device come from the cache;
+    only variants not timed before are run."""
+    p = tuning.sample_params(21, 4)[1]
+    a = gen.generate(p, stats=False)
+    dm = tuning.DeviceCSR(a, p.seed + 1)
+    cache = tuning.AutotuneCache(str(tmp_path / "tune.json"))
+    key = tuning.matrix_key(a, p)
+    first = tuning.autotune(dm, variants=[0, 4, 10], warmup=1, iters=3, cache=cache, key=key)
+    again = tuning.autotune(dm, variants=[4, 0], warmup=1, iters=3, cache=tuning.AutotuneCache(cache.path), key=key)
+    assert list(again) == [first[1], first[0]]
+    more = tuning.autotune(dm, variants=[10, 36], warmup=1, iters=3, cache=cache, key=key)
+    assert more[0] == first[2] and more[1] > 0
+    assert len(tuning.AutotuneCache(cache.path).get(key)) == 4

@@ -1,0 +1,51 @@
+"""End-to-end streaming (spmvdare/stream.py) on the first 16 matrices of the
+held-out stream (master seed 55: rows 1e4..4e6, avg 2..128 nnz/row, gamma
+rows, skew up to 1000): the committed selector chooses every matrix's format
+inside the batched launch, with its fallback to exhaustive tuning, and every
+batched y is checked against the oracle's csr_naive at 1e-13 normwise."""
+import os
+
+import numpy as np
+import pytest
+
+from spmvdare import gen, hip, stream, tuning
+import pyoracle
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MODEL = os.path.join(ROOT, "spmv-dare_amd", "spmvdare", "models", "selector.json")
+RANGES = dict(rows=(1e4, 4e6), avg=(2, 128), skew=(1, 1000), distributions=("gamma",))
+
+
+@pytest.fixture(scope="module")
+def selector():
+    hip.init(0)
+    return tuning.Selector.load(MODEL)
+
+
+def test_stream_first_16_matrices(selector):
+    params = tuning.sample_params(55, 64, **RANGES)
+    r = stream.run_stream(params, 16, 8, selector, tuned=2, fallback_margin=0.05, steps=3, keep_y=True,
+                          log=lambda s: print(s, flush=True))
+    print({k: v for k, v in r.items() if k != "y"}, flush=True)
+    assert r["matrices"] == 16 and len(r["y"]) == 16
+    assert r["max_normwise_err_vs_csr_naive"] <= 1e-13
+    assert sum(r["chosen_variants"].values()) == 16
+    assert r["regret"]["matrices"] == 2
+    for p, yb in r["y"]:
+        a = gen.generate(p, stats=False)
+        x = gen.make_x(a.n, p.seed + 1)
+        y_ref, _ = pyoracle.csr_naive_omp(a.row_ptr, a.col_ind, a.values, x)
+        err = pyoracle.normwise_rel_err(yb, y_ref)
+        print("stream member rows=%d nnz=%d: normwise vs csr_naive %.3e" % (a.m, a.nnz, err), flush=True)
+        assert err <= 1e-13
+
+
+def test_stream_fallback_is_exhaustive(selector):
+    """With a margin no prediction pair can beat, every choice falls back to
+    exhaustive tuning, so the policy's regret against that same tuning is 0."""
+    params = tuning.sample_params(21, 8)   # small matrices of the batched workload's ranges
+    r = stream.run_stream(params, 3, 3, selector, tuned=3, fallback_margin=1e9, steps=2)
+    assert r["fallbacks"] == 3
+    assert r["regret"]["policy_max"] == 0.0
+    assert r["max_normwise_err_vs_csr_naive"] <= 1e-13
