@@ -233,6 +233,22 @@ def probe(hip, np):
         out[name] = {"bytes": nbytes, "median_ms": float(np.median(ms)),
                      "gbps": nbytes / (float(np.median(ms)) * 1e-3) / 1e9}
         log("%s: %.3f ms, %.1f GB/s" % (name, out[name]["median_ms"], out[name]["gbps"]))
+    del src, dst
+    # the cost of one 64-lane load by pattern, on a buffer that stays in the
+    # L1 (16 KiB) and one that stays in the L2 (2 MiB): cycles per CU at 2.4 GHz
+    names = ["contiguous 4 B", "contiguous 8 B", "contiguous 16 B", "8 B gather, 64 lines",
+             "8 B gather, 64 lines, half the lanes", "16 B gather, 64 lines", "8 B gather, 16 lines",
+             "8 B gather, 4 lines"]
+    big = 1 << 18
+    buf = hip.DeviceArray.from_host(1.0 + (np.arange(big) & 7).astype(np.float64))
+    sink = hip.DeviceArray(1 << 17)
+    out["load_cost_cycles_per_cu"] = {}
+    for n, where in ((2048, "L1 (16 KiB)"), (big, "L2 (2 MiB)")):
+        row = {}
+        for mode, name in enumerate(names):
+            row[name] = round(hip.gather_probe(mode, buf, n, 4096, sink), 1)
+        out["load_cost_cycles_per_cu"][where] = row
+        log("load cost, %s: %s" % (where, row))
     print(json.dumps({"probe": out}), flush=True)
 
 
@@ -252,15 +268,23 @@ def run_e2e(args, gen, hip, np):
     sel = tuning.Selector.load(model)
     count = args.e2e_count
     params = tuning.sample_params(55, max(count * 2, count + 64), **E2E_RANGES)
+    draws = "first %d matrices" % count
+    if args.e2e_draws:
+        # a slice of the stream by draw index (every generatable matrix of
+        # draws [a, b)), so that a long stream can be run in several parts
+        a_, b_ = (int(v) for v in args.e2e_draws.split(":"))
+        params = tuning.sample_params(55, b_, **E2E_RANGES)[a_:b_]
+        count = b_ - a_
+        draws = "every generatable matrix of draws [%d, %d)" % (a_, b_)
     r = stream.run_stream(params, count, max(1, args.e2e_chunk), sel, tuned=args.e2e_tuned,
                           fallback_margin=args.fallback_margin, steps=args.e2e_steps, batch_bnnz=args.batch_bnnz,
-                          queues=args.queues, tile=args.tile, log=log)
+                          queues=args.queues, tile=args.tile, log=log, refine=args.refine)
     out = {
         "metric": "end-to-end batched FP64 SpMV matrices/s (selector-chosen formats), streaming",
         "value": round(r["matrices_per_s"], 1), "unit": "matrices/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f64", "data": "synthetic (reference generator restated bit-exactly)",
-        "config": {"workload": "first %d matrices of the master-seed-55 stream (rows 1e4..4e6, avg 2..128, gamma, "
-                               "skew <= 1000), chunks of %d" % (r["matrices"], r["chunk"]),
+        "config": {"workload": "%s (%d matrices) of the master-seed-55 stream (rows 1e4..4e6, avg 2..128, gamma, "
+                               "skew <= 1000), chunks of %d" % (draws, r["matrices"], r["chunk"]),
                    "nnz": r["nnz"], "alg_bytes": r["alg_bytes"]},
         "batched_ms": round(r["batched_ms"], 3), "batched_gbps": round(r["batched_gbps"], 1),
         "plain_per_matrix_csr": {"ms": round(r["plain_per_matrix_csr_ms"], 3),
@@ -268,6 +292,7 @@ def run_e2e(args, gen, hip, np):
         "speedup_vs_plain_csr": round(r["speedup_vs_plain_csr"], 3),
         "max_normwise_err_vs_csr_naive": r["max_normwise_err_vs_csr_naive"],
         "selector": {"fallback_margin": r["fallback_margin"], "fallbacks_to_exhaustive": r["fallbacks"],
+                     "refine_top_k": r["refine"],
                      "regret_vs_exhaustive": r["regret"]},
         "wall_s": round(r["wall_s"], 1), "chosen_variants": r["chosen_variants"],
     }
@@ -410,6 +435,8 @@ def main():
     ap.add_argument("--fallback-margin", type=float, default=0.05,
                     help="selector falls back to exhaustive tuning when its two best predictions are this close")
     ap.add_argument("--e2e-steps", type=int, default=10, help="timed launches per chunk")
+    ap.add_argument("--e2e-draws", default=None, help="A:B -- every generatable matrix of stream draws [A, B)")
+    ap.add_argument("--refine", type=int, default=3, help="time the k best predicted variants, keep the fastest")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=128)
     ap.add_argument("--warmup", type=int, default=10)

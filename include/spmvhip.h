@@ -255,6 +255,15 @@ void spmvhip_batch_destroy(spmvhip_batch *b);
  * iteration i's time. The measured ceiling for the SpMV kernels and the
  * known-size traffic that calibrates the profiler's byte counters. */
 int spmvhip_bandwidth_probe(int mode, const double *src, double *dst, long n, int iters, float *ms);
+/* The vector-memory cost of one 64-lane load, in cycles per CU at clock_ghz,
+ * by pattern (mode 0 contiguous 4-byte, 1 contiguous 8-byte, 2 contiguous
+ * 16-byte, 3 8-byte gather over 64 cache lines, 4 the same with half the lanes
+ * active, 5 16-byte gather over 64 lines, 6 8-byte gather over 16 lines, 7
+ * 8-byte gather over 4 lines) on the first n doubles of buf (n a power of two,
+ * >= 1024: small n stays in the L1, large in the L2). sink: >= 32 x CUs
+ * doubles. No reference counterpart (a measurement aid). */
+int spmvhip_gather_probe(int mode, const double *buf, long n, int iters, double clock_ghz, double *sink,
+		double *cycles_per_load);
 
 /* ---- format selector support (no counterpart in the reference) ---- */
 

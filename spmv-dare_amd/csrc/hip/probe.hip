@@ -4,6 +4,9 @@
 //   stream_read:  reads n doubles once (16-byte loads), writes one partial
 //                 sum per workgroup
 //   stream_copy:  dst[i] = src[i] over n doubles (16-byte loads and stores)
+//   gather_cost:  the vector-memory cost of one 64-lane load by width and
+//                 address pattern, on a cache-resident buffer (what the SpMV
+//                 kernels' x gathers pay per instruction)
 
 #include "format.hpp"
 
@@ -35,9 +38,111 @@ __global__ void __launch_bounds__(kBlock) stream_copy_kernel(const dbl2 *__restr
 		__builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
+// One wave's load pattern per iteration, every index masked into the buffer
+// (a power-of-two element count `mask + 1`): 0 contiguous 4-byte, 1 contiguous
+// 8-byte, 2 contiguous 16-byte, 3 8-byte gather over 64 lines, 4 the same
+// with half the lanes, 5 16-byte gather over 64 lines, 6 8-byte gather over
+// 16 lines (4 lanes per line), 7 8-byte gather over 4 lines (16 lanes per
+// 128-byte line).
+__device__ __forceinline__ uint32_t mix32(uint32_t a)
+{
+	a ^= a >> 16;
+	a *= 0x7feb352dU;
+	a ^= a >> 15;
+	a *= 0x846ca68bU;
+	a ^= a >> 16;
+	return a;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) gather_cost_kernel(const double *__restrict__ buf, uint32_t mask,
+		int iters, double *__restrict__ sink)
+{
+	const uint32_t lane = lane_id(), w = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+	double acc = 0;
+	uint32_t s = mix32(w * kWave + lane);
+#pragma unroll 8
+	for (int it = 0; it < iters; it++) {
+		const uint32_t base = (w * 131u + (uint32_t) it) * kWave;
+		if (MODE == 0) {
+			acc += ((const float *) buf)[(base + lane) & (2 * mask + 1)];
+		} else if (MODE == 1) {
+			acc += buf[(base + lane) & mask];
+		} else if (MODE == 2) {
+			const dbl2 v = ((const dbl2 *) buf)[(base + lane) & (mask >> 1)];
+			acc += v.x + v.y;
+		} else if (MODE == 3 || MODE == 4) {
+			s = s * 1664525u + 1013904223u;
+			if (MODE == 3 || (lane & 1))
+				acc += buf[(s >> 8) & mask];
+		} else if (MODE == 5) {
+			s = s * 1664525u + 1013904223u;
+			const dbl2 v = ((const dbl2 *) buf)[(s >> 8) & (mask >> 1)];
+			acc += v.x + v.y;
+		} else if (MODE == 6) {
+			acc += buf[(mix32(base + (lane >> 2)) * 16 + (lane & 3)) & mask];
+		} else {
+			acc += buf[(mix32(base + (lane >> 4)) * 16 + (lane & 15)) & mask];
+		}
+	}
+	if (acc == 12345.678) // never true for the probe's data; keeps the loads
+		sink[w] = acc;
+}
+
 } // namespace spmvhip
 
 using namespace spmvhip;
+
+// mode 0..7 (gather_cost_kernel), over the first n doubles of buf (n a power
+// of two, at least 1024): 8 workgroups of 4 waves per CU, each wave `iters`
+// loads; *cycles_per_load = kernel time x clock_ghz / loads per CU (after one
+// warm-up launch). sink: at least (8 x CUs x 4) doubles, never written for
+// the probe's data (buf filled with values in [1, 8]).
+extern "C" int spmvhip_gather_probe(int mode, const double *buf, long n, int iters, double clock_ghz, double *sink,
+		double *cycles_per_load)
+{
+	if (!buf || !sink || !cycles_per_load || n < 1024 || (n & (n - 1)) || n > (1L << 30) || iters < 1 ||
+			mode < 0 || mode > 7 || !(clock_ghz > 0))
+		return set_error(SPMVHIP_EINVAL, "gather_probe: bad arguments");
+	int dev = 0, cus = 0;
+	SPMVHIP_CHECK(hipGetDevice(&dev));
+	SPMVHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+	const int grid = cus * 8;
+	const uint32_t mask = (uint32_t) (n - 1);
+	hipEvent_t e0, e1;
+	SPMVHIP_CHECK(hipEventCreate(&e0));
+	SPMVHIP_CHECK(hipEventCreate(&e1));
+	float ms = 0;
+	hipError_t he = hipSuccess;
+	for (int rep = 0; rep < 2 && he == hipSuccess; rep++) {
+		he = hipEventRecord(e0, nullptr);
+		switch (mode) {
+		case 0: gather_cost_kernel<0><<<grid, kBlock>>>(buf, mask, iters, sink); break;
+		case 1: gather_cost_kernel<1><<<grid, kBlock>>>(buf, mask, iters, sink); break;
+		case 2: gather_cost_kernel<2><<<grid, kBlock>>>(buf, mask, iters, sink); break;
+		case 3: gather_cost_kernel<3><<<grid, kBlock>>>(buf, mask, iters, sink); break;
+		case 4: gather_cost_kernel<4><<<grid, kBlock>>>(buf, mask, iters, sink); break;
+		case 5: gather_cost_kernel<5><<<grid, kBlock>>>(buf, mask, iters, sink); break;
+		case 6: gather_cost_kernel<6><<<grid, kBlock>>>(buf, mask, iters, sink); break;
+		default: gather_cost_kernel<7><<<grid, kBlock>>>(buf, mask, iters, sink); break;
+		}
+		if (he == hipSuccess)
+			he = hipGetLastError();
+		if (he == hipSuccess)
+			he = hipEventRecord(e1, nullptr);
+		if (he == hipSuccess)
+			he = hipEventSynchronize(e1);
+		if (he == hipSuccess)
+			he = hipEventElapsedTime(&ms, e0, e1);
+	}
+	(void) hipEventDestroy(e0);
+	(void) hipEventDestroy(e1);
+	if (he != hipSuccess)
+		return set_error(SPMVHIP_EHIP, "gather_probe: %s", hipGetErrorString(he));
+	const double loads_per_cu = (double) grid * kWavesPerBlock * iters / cus;
+	*cycles_per_load = ms * 1e-3 * clock_ghz * 1e9 / loads_per_cu;
+	return SPMVHIP_OK;
+}
 
 // mode 0: stream read of src; mode 1: copy src -> dst. n doubles (even).
 // ms[i] = time of iteration i (HIP events).

@@ -87,6 +87,8 @@ _sig = {
     "spmvhip_csr_stats_device": ([_vp, _vp, ctypes.c_long, ctypes.c_long, _vp], ctypes.c_int),
     "spmvhip_csr_stats_device_noalloc": ([_vp, _vp, ctypes.c_long, ctypes.c_long, _vp, _vp], ctypes.c_int),
     "spmvhip_bandwidth_probe": ([ctypes.c_int, _vp, _vp, ctypes.c_long, ctypes.c_int, _vp], ctypes.c_int),
+    "spmvhip_gather_probe": ([ctypes.c_int, _vp, ctypes.c_long, ctypes.c_int, ctypes.c_double, _vp, _vp],
+                             ctypes.c_int),
     "spmvhip_mlp_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint,
                             ctypes.POINTER(_vp)], ctypes.c_int),
     "spmvhip_mlp_num_params": ([_vp], ctypes.c_long),
@@ -351,6 +353,15 @@ class Batch:
             self.destroy()
         except Exception:
             pass
+
+
+def gather_probe(mode, buf, n, iters, sink, clock_ghz=2.4):
+    """Cycles per CU of one 64-lane load of the given pattern (see
+    spmvhip_gather_probe) over the first n doubles of the device array buf."""
+    out = ctypes.c_double()
+    _check(_lib.spmvhip_gather_probe(mode, buf.ptr, n, iters, clock_ghz, sink.ptr, ctypes.byref(out)),
+           "gather_probe")
+    return out.value
 
 
 def bandwidth_probe(mode, src, dst, iters):

@@ -6,8 +6,9 @@ the host generates chunk i + 1 (a worker thread; the generator releases the
 interpreter lock) while the GPU takes chunk i: each matrix is uploaded, its
 structure statistics are computed on the device, the learned selector
 predicts every variant's slowdown, and the matrix is converted to the chosen
-variant. When the selector is unsure (its two best predictions lie within
-`fallback_margin` of each other), the choice falls back to exhaustive
+variant: the `refine` best predicted variants are timed and the fastest
+is kept, and when the selector is unsure (its two best predictions lie
+within `fallback_margin` of each other) the choice falls back to exhaustive
 autotuning over every variant the batched launch can run. The chunk then
 runs as one persistent batched launch replayed from a hipGraph, and is
 compared on the same GPU with plain per-matrix CSR-vector launches. Every
@@ -37,12 +38,13 @@ def _rel(y, yr):
     return float(np.linalg.norm(y - yr) / nrm) if nrm > 0 else float(np.linalg.norm(y))
 
 
-def choose(sel, dm, p, allowed, fallback_margin, tune, warmup=1, iters=5):
+def choose(sel, dm, p, allowed, fallback_margin, tune, refine=3, warmup=1, iters=5):
     """The variant a matrix runs as, and what was learned choosing it.
     Returns (variant, info): info["fallback"] when the selector's two best
     predictions were within fallback_margin (then the choice is the
-    exhaustive tuner's), info["times"] (exhaustive timings over `allowed`,
-    ms, NaN elsewhere) when tuned either way."""
+    exhaustive tuner's); otherwise the fastest of the `refine` best
+    predictions, timed (1: the top prediction untimed). info["times"]:
+    exhaustive timings over `allowed` (ms, NaN elsewhere) when tuned."""
     feats = tuning.features(p, dm.stats())
     pred = sel.predict(feats)[0].astype(np.float64)
     pred[~allowed] = np.inf
@@ -50,6 +52,7 @@ def choose(sel, dm, p, allowed, fallback_margin, tune, warmup=1, iters=5):
     margin = float(pred[order[1]] - pred[order[0]])
     fallback = margin < math.log1p(fallback_margin)
     info = {"top1": int(order[0]), "margin": margin, "fallback": bool(fallback)}
+    t = None
     if fallback or tune:
         idx = np.flatnonzero(allowed)
         t = np.full(len(tuning.VARIANTS), np.nan)
@@ -57,11 +60,16 @@ def choose(sel, dm, p, allowed, fallback_margin, tune, warmup=1, iters=5):
         info["times"] = t
         if fallback:
             return int(np.nanargmin(t)), info
-    return int(order[0]), info
+    top = [int(i) for i in order[:max(1, refine)] if np.isfinite(pred[i])]
+    if len(top) == 1:
+        return top[0], info
+    # the fastest of the best predictions (timings from the exhaustive run when there is one)
+    tt = t[top] if t is not None else tuning.autotune(dm, variants=top, warmup=warmup, iters=iters)
+    return top[int(np.argmin(tt))], info
 
 
 def run_stream(params, count, chunk, selector, tuned=0, fallback_margin=0.05, steps=10, batch_bnnz=256,
-               queues=256, tile=0, log=None, keep_y=False, solo_nnz=0):
+               queues=256, tile=0, log=None, keep_y=False, solo_nnz=0, refine=3):
     """Process the first `count` generatable matrices of `params` (a list of
     gen.GenParams) as described in the module docstring. Returns a dict of
     totals; with keep_y, also the list of (params, batched y) for checks."""
@@ -95,7 +103,7 @@ def run_stream(params, count, chunk, selector, tuned=0, fallback_margin=0.05, st
         mats = []
         for p, a in pending:
             dm = tuning.DeviceCSR(a, p.seed + 1)
-            v, info = choose(selector, dm, p, allowed, fallback_margin, done + len(mats) < tuned)
+            v, info = choose(selector, dm, p, allowed, fallback_margin, done + len(mats) < tuned, refine=refine)
             fallbacks += info["fallback"]
             if "times" in info and done + len(mats) < tuned:
                 t = info["times"]
@@ -156,7 +164,7 @@ def run_stream(params, count, chunk, selector, tuned=0, fallback_margin=0.05, st
         "speedup_vs_plain_csr": t_p / t_b if t_b else None,
         "max_normwise_err_vs_csr_naive": worst,
         "max_normwise_err_vs_csr_naive_plain_csr": worst_plain,
-        "fallback_margin": fallback_margin, "fallbacks": fallbacks,
+        "fallback_margin": fallback_margin, "fallbacks": fallbacks, "refine": refine,
         "regret": {"matrices": len(regrets),
                    "top1_mean": float(r[:, 0].mean()) if len(r) else None,
                    "top1_max": float(r[:, 0].max()) if len(r) else None,
