@@ -228,7 +228,7 @@ def probe(hip, np):
     dst = hip.DeviceArray(n)
     src.fill_bytes(0)
     out = {}
-    for mode, name, nbytes in ((0, "stream_read", 8 * n), (1, "stream_copy", 16 * n)):
+    for mode, name, nbytes in ((0, "stream_read", 8 * n), (1, "stream_copy", 16 * n), (2, "stream_read_lds", 8 * n)):
         ms = hip.bandwidth_probe(mode, src, dst if mode == 1 else None, 20)
         out[name] = {"bytes": nbytes, "median_ms": float(np.median(ms)),
                      "gbps": nbytes / (float(np.median(ms)) * 1e-3) / 1e9}
@@ -238,7 +238,8 @@ def probe(hip, np):
     # L1 (16 KiB) and one that stays in the L2 (2 MiB): cycles per CU at 2.4 GHz
     names = ["contiguous 4 B", "contiguous 8 B", "contiguous 16 B", "8 B gather, 64 lines",
              "8 B gather, 64 lines, half the lanes", "16 B gather, 64 lines", "8 B gather, 16 lines",
-             "8 B gather, 4 lines"]
+             "8 B gather, 4 lines", "8 B buffer gather, 64 lines", "the same, sc0", "the same, nt",
+             "the same, sc1", "the same, sc0 sc1", "the same, sc0 sc1 nt"]
     big = 1 << 18
     buf = hip.DeviceArray.from_host(1.0 + (np.arange(big) & 7).astype(np.float64))
     sink = hip.DeviceArray(1 << 17)

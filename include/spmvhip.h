@@ -251,7 +251,8 @@ int spmvhip_batch_info(const spmvhip_batch *b, long *nmats, long *nitems, int *g
 void spmvhip_batch_destroy(spmvhip_batch *b);
 
 /* Bandwidth reference (no counterpart in the reference): mode 0 streams n
- * doubles of src once (16-byte loads), mode 1 copies src to dst; ms[i] is
+ * doubles of src once (16-byte loads), mode 1 copies src to dst, mode 2
+ * streams src through direct global-to-LDS loads; ms[i] is
  * iteration i's time. The measured ceiling for the SpMV kernels and the
  * known-size traffic that calibrates the profiler's byte counters. */
 int spmvhip_bandwidth_probe(int mode, const double *src, double *dst, long n, int iters, float *ms);
@@ -259,8 +260,10 @@ int spmvhip_bandwidth_probe(int mode, const double *src, double *dst, long n, in
  * by pattern (mode 0 contiguous 4-byte, 1 contiguous 8-byte, 2 contiguous
  * 16-byte, 3 8-byte gather over 64 cache lines, 4 the same with half the lanes
  * active, 5 16-byte gather over 64 lines, 6 8-byte gather over 16 lines, 7
- * 8-byte gather over 4 lines) on the first n doubles of buf (n a power of two,
- * >= 1024: small n stays in the L1, large in the L2). sink: >= 32 x CUs
+ * 8-byte gather over 4 lines, 8..13 mode 3 as buffer loads with the cache
+ * policy none, sc0, nt, sc1, sc0 sc1, sc0 sc1 nt; n <= 2^27 for those) on the
+ * first n doubles of buf (n a power of two, >= 1024: small n stays in the L1,
+ * large in the L2). sink: >= 32 x CUs
  * doubles. No reference counterpart (a measurement aid). */
 int spmvhip_gather_probe(int mode, const double *buf, long n, int iters, double clock_ghz, double *sink,
 		double *cycles_per_load);

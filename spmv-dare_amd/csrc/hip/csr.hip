@@ -63,6 +63,21 @@ int csr_naive_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 
 constexpr int kMaxBlockRows = 64;
 
+// A row block's rows [r0, r1) and nonzeros [j0, j1).
+struct alignas(16) BlockRec {
+	int32_t r0, r1, j0, j1;
+};
+
+__global__ void csr_fill_records(const int32_t *__restrict__ blocks, const int32_t *__restrict__ rp, long nb,
+		BlockRec *__restrict__ out)
+{
+	const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nb)
+		return;
+	const int32_t r0 = blocks[i], r1 = blocks[i + 1];
+	out[i] = BlockRec{r0, r1, rp[r0], rp[r1]};
+}
+
 // flag[r] = 1 where row r starts a row block; split[r] = 1 for split rows.
 __global__ void csr_mark_blocks(const int32_t *__restrict__ rp, long m, int bnnz, int split_nnz,
 		int64_t *__restrict__ flag, int64_t *__restrict__ split)
@@ -169,6 +184,14 @@ int csr_vec_convert(spmvhip_format *f)
 			f->blocks.as<int32_t>(), split.as<int64_t>(), spos.as<int64_t>(),
 			ns > 0 ? f->split_rows.as<int32_t>() : nullptr);
 	SPMVHIP_CHECK(hipGetLastError());
+	// one 16-byte record per block: a wave reads its block's rows and
+	// nonzero range with one scalar load
+	SPMVHIP_TRY(f->brec.alloc((nb > 0 ? nb : 1) * sizeof(BlockRec)));
+	if (nb > 0) {
+		csr_fill_records<<<grid_for(nb, kBlock), kBlock, 0, s>>>(f->blocks.as<int32_t>(), f->row_ptr.as<int32_t>(),
+				nb, f->brec.as<BlockRec>());
+		SPMVHIP_CHECK(hipGetLastError());
+	}
 
 	if (ns > 0) {
 		DevBuf cnt, segpos;
@@ -192,27 +215,33 @@ int csr_vec_convert(spmvhip_format *f)
 		SPMVHIP_CHECK(hipStreamSynchronize(s));
 	}
 	SPMVHIP_CHECK(hipStreamSynchronize(s));
-	f->mem_footprint = f->csr_mem_footprint + (double) (nb + 1) * 4 + (double) f->nsegs * 20;
+	f->mem_footprint = f->csr_mem_footprint + (double) (nb + 1) * 4 + (double) nb * sizeof(BlockRec) +
+			(double) f->nsegs * 20;
 	return SPMVHIP_OK;
 }
 
 // ------------------------------------------------------------------ binned kernels
 
+// One wave per row block. The block's bounds come from its 16-byte record
+// with one scalar load (through the constant address space), so the wave's
+// stream is requested one memory round trip after it starts, not two (block
+// table, then row pointers).
 template <int BNNZ_MAX, bool NARROW>
 __global__ void __launch_bounds__(kBlock) csr_binned_kernel(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
 		const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
-		const int32_t *__restrict__ blocks, long nblocks, int split_nnz, int lanes_cfg, int remap)
+		const BlockRec *__restrict__ brec, long nblocks, int split_nnz, int lanes_cfg, int remap)
 {
 	__shared__ double prod[kWavesPerBlock][2 * BNNZ_MAX];
 	const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); // wave-uniform: scalar loads below
 	const long w = xcd_logical_block(remap) * kWavesPerBlock + wv;
 	if (w >= nblocks)
 		return; // waves never synchronise with each other: early exit is safe
-	csr_block<BNNZ_MAX, NARROW>(rp, col, val, x, y, blocks[w], blocks[w + 1], split_nnz, true, lanes_cfg,
+	const BlockRec r = const_ptr<BlockRec>(ptr_bits(brec))[w];
+	csr_block_at<BNNZ_MAX, NARROW>(rp, col, val, x, y, r.r0, r.r1, r.j0, r.j1, split_nnz, true, lanes_cfg,
 			prod[wv]);
 }
 
-// Persistent variant: a grid of resident workgroups whose waves take blocks
+// Persistent variant: a grid of resident workgroups whose waves take blocks// Persistent variant: a grid of resident workgroups whose waves take blocks
 // w, w + W, w + 2W, ... (W waves in all, so at any time the waves work on W
 // consecutive blocks), with block w + W's stream requested while block w is
 // reduced (kernels.hpp: csr_stage_*).
@@ -316,7 +345,7 @@ int csr_vec_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s)
 	} else {
 #define LAUNCH_BINNED(B, N)                                                                                 \
 	csr_binned_kernel<B, N><<<grid, kBlock, 0, s>>>(f->row_ptr.as<int32_t>(), f->col.as<int32_t>(),     \
-			f->val.as<double>(), x, y, f->blocks.as<int32_t>(), f->nblocks, f->split_nnz, f->lanes, \
+			f->val.as<double>(), x, y, f->brec.as<BlockRec>(), f->nblocks, f->split_nnz, f->lanes, \
 			remap)
 		if (f->block_nnz <= 256) {
 			if (narrow)

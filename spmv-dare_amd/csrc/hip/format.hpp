@@ -18,6 +18,7 @@ struct spmvhip_format {
 
 	// CSR_VEC: row blocks (row starts, nblocks + 1 entries) and split rows
 	spmvhip::DevBuf blocks;
+	spmvhip::DevBuf brec;         // int32 x 4 per block: (first row, end row, first nnz, end nnz)
 	long nblocks = 0;
 	int block_nnz = 512, split_nnz = 8192, lanes = 0;
 	unsigned pipe_grid = 0;       // resident workgroups of the persistent CSR kernel (set on first use)
