@@ -115,7 +115,12 @@ typedef struct spmvhip_params {
 	                         div_kernels_rf.h:467-510): every value rounded to keep
 	                         ceil(|log2 tolerance|) fraction bits, relative error <= tolerance
 	                         (< 1); 0 (default) = lossless */
-	int reserved[2];
+	int div_encoder;      /* DIV_RF: where the packets are encoded. 0 (default) = on the device when
+	                         the packets hold at most 2048 values, the lanes are at most 64, the
+	                         layout is 0, 1 or 3 and there is no packet reordering, else on the
+	                         host; 1 = host; 2 = device (an error when not possible). The bytes
+	                         are the same either way */
+	int reserved[1];
 } spmvhip_params;
 
 typedef struct spmvhip_format spmvhip_format;   /* opaque */
@@ -198,7 +203,9 @@ int spmvhip_time_spmv(spmvhip_format *f, const double *d_x, double *d_y,
  *        8 DIV decoded values (float64, nnz) -- device decode, packet order [DIV_RF]
  *        9 x windows (int32 pairs lo, width per workgroup of 4 slices; width 0 =
  *          not staged)                     [SELL, SELL_C_S with x_window]
- * *bytes_inout: capacity in, bytes written out (or needed, with buf NULL). */
+ *        10 packet bytes in the device layout (uint8, the packets without the
+ *          trailing padding), 11 their offsets (int64, npackets + 1) [DIV_RF] */
+/* *bytes_inout: capacity in, bytes written out (or needed, with buf NULL). */
 int spmvhip_export(const spmvhip_format *f, int which, void *buf, size_t *bytes_inout);
 
 void spmvhip_destroy(spmvhip_format *f);

@@ -315,26 +315,14 @@ int reduce_precision(const double *in, long n, double tolerance, double *out, co
 	return 0;
 }
 
-int encode(const int32_t *rp, const int32_t *col, const double *val, long m, long n, long nnz, int T, int L,
-		long P, Stream *out, const char **err)
+void plan_packets(const int32_t *rp, long m, long nnz, int T, long P, PacketPlan *out)
 {
-	(void) n;
-	*err = nullptr;
-	out->T = T;
-	out->L = L;
-	out->P = P;
-	out->bytes.clear();
-	out->pk_off.assign(1, 0);
-	out->pk_nnz.clear();
+	out->i0.clear();
+	out->j0.clear();
+	out->nv.clear();
 	out->grp_pk.clear();
 	out->grp_row.clear();
 	out->part_grp.clear();
-
-	struct Plan {
-		long i, j, nv;
-		bool group_start;
-	};
-	std::vector<Plan> plan;
 	std::vector<long> grp_first_row;
 	for (long t = 0; t < T; t++) {
 		out->part_grp.push_back((int32_t) grp_first_row.size());
@@ -354,22 +342,42 @@ int encode(const int32_t *rp, const int32_t *col, const double *val, long m, lon
 				i++;
 			long nmax = (j + P <= j_e) ? P : j_e - j;
 			long nv = select_num_vals(rp, i, j, nmax);
-			bool gs = (j == rp[i]);
-			plan.push_back({i, j, nv, gs});
-			if (gs)
+			if (j == rp[i]) { // a packet that starts a row starts a packet group
+				out->grp_pk.push_back((int32_t) out->i0.size());
 				grp_first_row.push_back(first ? i_s : i);
+			}
+			out->i0.push_back((int32_t) i);
+			out->j0.push_back(j);
+			out->nv.push_back((int32_t) nv);
 			first = false;
 			j += nv;
 		}
 	}
-	const long np = (long) plan.size();
+	out->grp_pk.push_back((int32_t) out->i0.size());
+	for (size_t g = 0; g < grp_first_row.size(); g++)
+		out->grp_row.push_back(g == 0 ? 0 : (int32_t) grp_first_row[g]);
+	out->grp_row.push_back((int32_t) m);
+	out->part_grp.push_back((int32_t) grp_first_row.size());
+}
+
+int encode(const int32_t *rp, const int32_t *col, const double *val, long m, long n, long nnz, int T, int L,
+		long P, Stream *out, const char **err)
+{
+	(void) n;
+	*err = nullptr;
+	out->T = T;
+	out->L = L;
+	out->P = P;
+	PacketPlan plan;
+	plan_packets(rp, m, nnz, T, P, &plan);
+	const long np = (long) plan.i0.size();
 	std::vector<std::vector<uint8_t>> pk(np);
 	int fail = 0;
 	const char *ferr = nullptr;
 #pragma omp parallel for schedule(dynamic, 8)
 	for (long p = 0; p < np; p++) {
 		const char *e = nullptr;
-		if (compress(rp, col, val, plan[p].i, plan[p].j, plan[p].nv, L, pk[p], &e)) {
+		if (compress(rp, col, val, plan.i0[p], plan.j0[p], plan.nv[p], L, pk[p], &e)) {
 #pragma omp critical
 			{
 				fail = 1;
@@ -385,22 +393,19 @@ int encode(const int32_t *rp, const int32_t *col, const double *val, long m, lon
 	for (long p = 0; p < np; p++)
 		total += pk[p].size();
 	out->bytes.resize(total);
+	out->pk_off.assign(1, 0);
+	out->pk_nnz.assign(plan.j0.begin(), plan.j0.end());
+	out->pk_nnz.push_back(nnz);
 	size_t at = 0;
 	for (long p = 0; p < np; p++) {
 		if (!pk[p].empty())
 			memcpy(&out->bytes[at], pk[p].data(), pk[p].size());
 		at += pk[p].size();
 		out->pk_off.push_back((int64_t) at);
-		out->pk_nnz.push_back(plan[p].j);
-		if (plan[p].group_start)
-			out->grp_pk.push_back((int32_t) p);
 	}
-	out->pk_nnz.push_back(nnz);
-	out->grp_pk.push_back((int32_t) np);
-	for (size_t g = 0; g < grp_first_row.size(); g++)
-		out->grp_row.push_back(g == 0 ? 0 : (int32_t) grp_first_row[g]);
-	out->grp_row.push_back((int32_t) m);
-	out->part_grp.push_back((int32_t) grp_first_row.size());
+	out->grp_pk.swap(plan.grp_pk);
+	out->grp_row.swap(plan.grp_row);
+	out->part_grp.swap(plan.part_grp);
 	return 0;
 }
 

@@ -38,6 +38,22 @@ struct Stream {
 	long ngroups() const { return (long) grp_pk.size() - 1; }
 };
 
+// The packets of a CSR matrix before encoding (div.cpp:236-513): for every
+// packet its first row (the row holding its first nonzero, empty rows
+// skipped), first nonzero and value count, in partition order; the packet
+// groups (a group starts with every packet that starts a row) with their
+// first packet and first owned row; the first group of each partition.
+// Packet boundaries chain, so the plan is built sequentially per partition.
+struct PacketPlan {
+	std::vector<int32_t> i0;
+	std::vector<int64_t> j0;
+	std::vector<int32_t> nv;
+	std::vector<int32_t> grp_pk;   // first packet of each group, + npackets
+	std::vector<int32_t> grp_row;  // first row owned by each group, + m
+	std::vector<int32_t> part_grp; // first group of each partition, + ngroups
+};
+void plan_packets(const int32_t *row_ptr, long m, long nnz, int T, long P, PacketPlan *out);
+
 // Encode a CSR matrix. Returns 0, or -1 with *err set (row span > 255 or
 // column span > 32 bits in a packet, which the reference rejects with error()).
 int encode(const int32_t *row_ptr, const int32_t *col, const double *val, long m, long n, long nnz,

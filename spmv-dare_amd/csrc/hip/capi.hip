@@ -306,17 +306,6 @@ int spmvhip_csr_to_format_device(const int32_t *d_row_ptr, const int32_t *d_col_
 {
 	if (!d_row_ptr || (nnz > 0 && (!d_col_ind || !d_values)))
 		return set_error(SPMVHIP_EINVAL, "NULL CSR array");
-	if (params && params->kind == SPMVHIP_DIV_RF) {
-		// the DIV encoder runs on the host: bring the CSR over and take the host path
-		std::vector<int32_t> rp(m + 1), col(nnz);
-		std::vector<double> val(nnz);
-		SPMVHIP_CHECK(hipMemcpy(rp.data(), d_row_ptr, (m + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
-		if (nnz) {
-			SPMVHIP_CHECK(hipMemcpy(col.data(), d_col_ind, nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
-			SPMVHIP_CHECK(hipMemcpy(val.data(), d_values, nnz * sizeof(double), hipMemcpyDeviceToHost));
-		}
-		return spmvhip_csr_to_format(rp.data(), col.data(), val.data(), m, n, nnz, 0, 0, params, out);
-	}
 	spmvhip_format *f = nullptr;
 	SPMVHIP_TRY(create_handle(m, n, nnz, params, &f));
 	int rc = alloc_csr(f);
@@ -537,6 +526,13 @@ int spmvhip_export(const spmvhip_format *f, int which, void *buf, size_t *bytes_
 		if (f->kind != SPMVHIP_DIV_RF)
 			return set_error(SPMVHIP_EINVAL, "decoded streams exist only for DIV_RF");
 		return div_export_decoded(f, which, buf, bytes_inout);
+	case 10:
+	case 11:
+		if (f->kind != SPMVHIP_DIV_RF)
+			return set_error(SPMVHIP_EINVAL, "packet streams exist only for DIV_RF");
+		src = which == 10 ? f->div_stream.p : f->div_pk_off.p;
+		need = which == 10 ? (size_t) f->div_bytes : (size_t) (f->div_npackets + 1) * sizeof(int64_t);
+		break;
 	case 9:
 		if (!sell || f->x_cap == 0)
 			return set_error(SPMVHIP_EINVAL, "x windows exist only for SELL formats built with x_window");

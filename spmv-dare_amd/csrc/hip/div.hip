@@ -1112,10 +1112,85 @@ __global__ void fill_zero(double *y, long m)
 		y[i] = 0.0;
 }
 
+// The LDS row accumulators of the staged decoders for packets of up to
+// max_rows rows: an odd stride puts the copies of one row in different LDS
+// banks; the automatic copy count is the largest power of two up to
+// kDivAccCopiesAuto within kDivAccBudget bytes.
+static void div_acc_setup(spmvhip_format *f, uint32_t max_rows)
+{
+	f->div_acc_stride = (int) (max_rows | 1);
+	int nc = f->p.div_acc_copies;
+	if (nc == 0) {
+		nc = 1;
+		while (nc < kDivAccCopiesAuto && 2 * nc * f->div_acc_stride * (long) sizeof(double) <= kDivAccBudget)
+			nc *= 2;
+	}
+	f->p.div_acc_copies = nc;
+	f->div_acc_bytes = (int) ((nc * f->div_acc_stride * sizeof(double) + 15) / 16 * 16);
+}
+
+// div_rf conversion with the packets encoded on the device (div_enc.hip):
+// the packet plan from the host row pointers, the bytes from the device CSR,
+// in the automatic layout (interleaved lane streams when a packet fits the
+// LDS, else the reference bytes) or the requested one (1 or 3).
+static int div_convert_device(spmvhip_format *f, const int32_t *h_rp, int T, int L, long P)
+{
+	spmvdiv::PacketPlan plan;
+	spmvdiv::plan_packets(h_rp, f->m, f->nnz, T, P, &plan);
+	const long np = (long) plan.i0.size(), ng = (long) plan.grp_pk.size() - 1;
+	f->div_order_host.clear();
+	f->div_npackets = np;
+	f->div_ngroups = ng;
+	DivDeviceEncoder enc;
+	SPMVHIP_TRY(enc.prepare(f, plan, L));
+	const int want = f->p.div_layout;
+	bool il = want == 0 || want == 3;
+	int64_t mx = 0;
+	uint32_t max_rows = 1;
+	SPMVHIP_TRY(enc.size(il, &mx, &max_rows));
+	div_acc_setup(f, max_rows);
+	if (il && mx + 16 + 8L * L + f->div_acc_bytes > kDivLdsMax) {
+		if (want == 3)
+			return set_error(SPMVHIP_EINVAL, "div_rf: layout 3 needs %ld bytes of LDS per packet (at most %ld)",
+					(long) (mx + 16 + 8L * L + f->div_acc_bytes), kDivLdsMax);
+		il = false;
+		SPMVHIP_TRY(enc.size(il, &mx, &max_rows));
+	}
+	SPMVHIP_TRY(enc.write());
+	f->div_slots = false;
+	f->div_il = il;
+	f->div_xs_bytes = 0;
+	const int64_t max_pk = mx + (il ? 8L * L : 0);
+	f->div_buf_bytes = max_pk + 16 + f->div_acc_bytes <= kDivLdsMax ? (int) (max_pk + 16) : 0;
+	std::vector<int64_t> pk_nnz(plan.j0.begin(), plan.j0.end());
+	pk_nnz.push_back(f->nnz);
+	SPMVHIP_TRY(f->div_pk_nnz.alloc((np + 1) * sizeof(int64_t)));
+	SPMVHIP_TRY(f->div_groups.alloc((ng + 1) * sizeof(int32_t)));
+	SPMVHIP_TRY(f->div_group_rows.alloc((ng + 1) * sizeof(int32_t)));
+	hipStream_t s = f->stream;
+	SPMVHIP_CHECK(hipMemcpyAsync(f->div_pk_nnz.p, pk_nnz.data(), (np + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+	SPMVHIP_CHECK(hipMemcpyAsync(f->div_groups.p, plan.grp_pk.data(), (ng + 1) * sizeof(int32_t),
+			hipMemcpyHostToDevice, s));
+	SPMVHIP_CHECK(hipMemcpyAsync(f->div_group_rows.p, plan.grp_row.data(), (ng + 1) * sizeof(int32_t),
+			hipMemcpyHostToDevice, s));
+	SPMVHIP_CHECK(hipStreamSynchronize(s));
+	f->row_ptr.release();
+	f->col.release();
+	f->val.release();
+	f->mem_footprint = (double) f->div_bytes + (double) (np + 1) * 16 + (double) (ng + 1) * 8;
+	return SPMVHIP_OK;
+}
+
 int div_convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col, const double *h_val)
 {
-	if (!h_rp)
-		return set_error(SPMVHIP_EINVAL, "div_rf: host CSR arrays are required");
+	// the packet plan reads the row pointers on the host (the CSR of a
+	// device-array conversion is brought over as far as it is needed)
+	std::vector<int32_t> hc_rp;
+	if (!h_rp) {
+		hc_rp.resize(f->m + 1);
+		SPMVHIP_CHECK(hipMemcpy(hc_rp.data(), f->row_ptr.p, (f->m + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+		h_rp = hc_rp.data();
+	}
 	const int T = f->p.div_threads > 0 ? f->p.div_threads : 1;
 	const int L = f->p.div_lanes > 0 ? f->p.div_lanes : 64;
 	const long P = f->p.div_packet_vals > 0 ? f->p.div_packet_vals : 16384;
@@ -1131,6 +1206,33 @@ int div_convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col, co
 	f->p.div_lanes = L;
 	f->p.div_packet_vals = P;
 	f->div_lanes = L;
+	if (f->p.div_reorder < 0 || f->p.div_reorder > 1)
+		return set_error(SPMVHIP_EINVAL, "div_rf: div_reorder %d (0 stream order, 1 by median column)",
+				f->p.div_reorder);
+	if (f->p.div_layout < 0 || f->p.div_layout > 4)
+		return set_error(SPMVHIP_EINVAL, "div_rf: div_layout %d (0 automatic, 1 reference bytes, 2 x slots, "
+				"3 interleaved lane streams, 4 x slots and interleaved lane streams)", f->p.div_layout);
+	const int enc = f->p.div_encoder;
+	if (enc < 0 || enc > 2)
+		return set_error(SPMVHIP_EINVAL, "div_rf: div_encoder %d (0 automatic, 1 host, 2 device)", enc);
+	const bool dev_ok = div_device_encodable(P, L) && f->p.div_reorder == 0 &&
+			(f->p.div_layout == 0 || f->p.div_layout == 1 || f->p.div_layout == 3);
+	if (enc == 2 && !dev_ok)
+		return set_error(SPMVHIP_EINVAL, "div_rf: the device encoder takes packets of at most 2048 values, at "
+				"most 64 lanes, layouts 1 and 3 and no packet reordering");
+	if (enc != 1 && dev_ok && f->nnz > 0)
+		return div_convert_device(f, h_rp, T, L, P);
+	// the host encoder reads the CSR on the host
+	std::vector<int32_t> hc_col;
+	std::vector<double> hc_val;
+	if (!h_col && f->nnz > 0) {
+		hc_col.resize(f->nnz);
+		hc_val.resize(f->nnz);
+		SPMVHIP_CHECK(hipMemcpy(hc_col.data(), f->col.p, f->nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+		SPMVHIP_CHECK(hipMemcpy(hc_val.data(), f->val.p, f->nnz * sizeof(double), hipMemcpyDeviceToHost));
+		h_col = hc_col.data();
+		h_val = hc_val.data();
+	}
 	spmvdiv::Stream st;
 	const char *err = nullptr;
 	std::vector<double> lossy;
@@ -1143,15 +1245,9 @@ int div_convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col, co
 	if (spmvdiv::encode(h_rp, h_col, h_val, f->m, f->n, f->nnz, T, L, P, &st, &err))
 		return set_error(SPMVHIP_EFORMAT, "div_rf: %s (the reference rejects this matrix too)", err);
 	const long np = st.npackets(), ng = st.ngroups();
-	if (f->p.div_reorder < 0 || f->p.div_reorder > 1)
-		return set_error(SPMVHIP_EINVAL, "div_rf: div_reorder %d (0 stream order, 1 by median column)",
-				f->p.div_reorder);
 	f->div_order_host.clear();
 	if (f->p.div_reorder && ng > 0)
 		f->div_order_host = spmvdiv::group_order_by_median(st, h_col);
-	if (f->p.div_layout < 0 || f->p.div_layout > 4)
-		return set_error(SPMVHIP_EINVAL, "div_rf: div_layout %d (0 automatic, 1 reference bytes, 2 x slots, "
-				"3 interleaved lane streams, 4 x slots and interleaved lane streams)", f->p.div_layout);
 	f->div_npackets = np;
 	f->div_ngroups = ng;
 	// the most rows of any packet (header word 2, at most 255)
@@ -1161,16 +1257,7 @@ int div_convert(spmvhip_format *f, const int32_t *h_rp, const int32_t *h_col, co
 		memcpy(&nr, &st.bytes[st.pk_off[p] + 8], sizeof(nr));
 		max_rows = std::max(max_rows, nr);
 	}
-	// an odd stride puts the copies of one row in different LDS banks
-	f->div_acc_stride = (int) (max_rows | 1);
-	int nc = f->p.div_acc_copies;
-	if (nc == 0) {
-		nc = 1;
-		while (nc < kDivAccCopiesAuto && 2 * nc * f->div_acc_stride * (long) sizeof(double) <= kDivAccBudget)
-			nc *= 2;
-	}
-	f->p.div_acc_copies = nc;
-	f->div_acc_bytes = (int) ((nc * f->div_acc_stride * sizeof(double) + 15) / 16 * 16);
+	div_acc_setup(f, max_rows);
 	// the device layout: the requested one, or (automatic) the first of
 	// kDivAutoLayouts whose packet (+ 16 of zero padding, + 2 rows of stream
 	// slack when interleaved), staged x slots and row accumulators fit the LDS

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "../div/div_codec.hpp"
 
 #include <vector>
 
@@ -109,6 +110,30 @@ struct BatchDivUnit {
 // kernel cannot run (packets decoded from global memory, x-slot layout).
 int div_batch_member(const spmvhip_format *f, const double *d_x, double *d_y, BatchDivMat *out, int *buf_bytes,
 		int *acc_bytes);
+// DIV-RF encoding on the device (div_enc.hip): packets of at most 2048
+// values and at most 64 lanes.
+bool div_device_encodable(long P, int L);
+// The device encoder of one handle: prepare() uploads the packet plan (and
+// rounds lossy values on the device), size() computes every packet's bytes
+// in a device layout (il: interleaved lane streams, else the reference
+// bytes), write() lays the packets out for the last size() into the handle's
+// div_stream / div_pk_off (and div_bytes).
+class DivDeviceEncoder {
+public:
+	~DivDeviceEncoder();
+	int prepare(spmvhip_format *f, const spmvdiv::PacketPlan &plan, int L);
+	int size(bool il, int64_t *max_packet, uint32_t *max_rows);
+	int write();
+
+private:
+	int launch(bool write, bool il, const int64_t *off, uint8_t *out);
+	spmvhip_format *f_ = nullptr;
+	int L_ = 64;
+	long np_ = 0;
+	bool sized_il_ = false;
+	const double *vals_ = nullptr;
+	DevBuf lossy_, i0_, j0_, nv_, info_, size_;
+};
 // Resident workgroups per CU of the batched DIV kernel with that much LDS.
 int div_batch_occupancy(int lds_bytes, int *per_cu);
 // Enqueues the counter reset (nc counters, `stride` ints apart) and the

@@ -32,7 +32,8 @@ class Params(ctypes.Structure):
                 ("div_packet_vals", ctypes.c_long), ("pipeline", ctypes.c_int),
                 ("y_mode", ctypes.c_int), ("div_acc_copies", ctypes.c_int), ("narrow_loads", ctypes.c_int),
                 ("div_layout", ctypes.c_int), ("div_reorder", ctypes.c_int),
-                ("div_tolerance", ctypes.c_double), ("reserved", ctypes.c_int * 2)]
+                ("div_tolerance", ctypes.c_double), ("div_encoder", ctypes.c_int),
+                ("reserved", ctypes.c_int * 1)]
 
 
 class Info(ctypes.Structure):
