@@ -319,20 +319,32 @@ __device__ __forceinline__ uint32_t lds_le_b32x(const uint8_t *base, int pos, ui
 // LROW is lrow when known at compile time), so the lanes' three dword reads
 // fall in distinct banks; the reads past a stream's end stay inside the
 // packet buffer's two rows of slack. Otherwise the stream is contiguous.
+//
+// The low nbytes bytes are kept with one 64-bit mask, ~((~0 << 4n) << 4n)
+// (two shifts of at most 32, so n = 0 and n = 8 need no special case):
+// about half the instructions of separate 32-bit masks.
 template <bool IL, int LROW>
 __device__ __forceinline__ uint64_t stream_read(const uint8_t *pk, int sbase, int cursor, int nbytes, int lrow)
 {
-	if (!IL)
-		return lds_le_b32(pk, sbase + cursor, nbytes);
-	const int row = LROW ? LROW : lrow;
-	const uint32_t *w = (const uint32_t *) (pk + sbase + (cursor >> 2) * row);
-	const uint32_t sh = (uint32_t) (cursor & 3);
-	const uint32_t w0 = w[0], w1 = w[row / 4], w2 = w[row / 2];
-	uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, sh);
-	uint32_t hi = __builtin_amdgcn_alignbyte(w2, w1, sh);
-	lo &= nbytes >= 4 ? 0xffffffffu : (1u << (8 * nbytes)) - 1;
-	hi &= nbytes >= 8 ? 0xffffffffu : nbytes > 4 ? (1u << (8 * (nbytes - 4))) - 1 : 0u;
-	return ((uint64_t) hi << 32) | lo;
+	uint32_t lo, hi;
+	if (IL) {
+		const int row = LROW ? LROW : lrow;
+		const uint32_t *w = (const uint32_t *) (pk + sbase + (cursor >> 2) * row);
+		const uint32_t sh = (uint32_t) (cursor & 3);
+		const uint32_t w0 = w[0], w1 = w[row / 4], w2 = w[row / 2];
+		lo = __builtin_amdgcn_alignbyte(w1, w0, sh);
+		hi = __builtin_amdgcn_alignbyte(w2, w1, sh);
+	} else {
+		const int pos = sbase + cursor;
+		const uint32_t *w = (const uint32_t *) (pk + (pos & ~3));
+		const uint32_t sh = (uint32_t) (pos & 3);
+		const uint32_t w0 = w[0], w1 = w[1], w2 = w[2];
+		lo = __builtin_amdgcn_alignbyte(w1, w0, sh);
+		hi = __builtin_amdgcn_alignbyte(w2, w1, sh);
+	}
+	const uint64_t v = ((uint64_t) hi << 32) | lo;
+	const uint32_t s4 = 4u * (uint32_t) nbytes;
+	return v & ~((~0ull << s4) << s4);
 }
 
 

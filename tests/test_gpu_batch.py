@@ -214,3 +214,34 @@ def test_batch_rejects_slot_layout_div():
     with pytest.raises(hip.SpmvHipError):
         b.add(f, dx, dy)
     b.destroy()
+
+
+@pytest.mark.parametrize("ranges", [False, True])
+def test_batch_empty_blocks_at_odd_offsets(members, ranges):
+    """Empty row blocks that start at an odd nonzero index (a row of 300
+    entries, then an empty row, which starts a block of its own): the paired
+    stream loads still run once for them, and the wave must read its next
+    ticket exactly once per unit, or a unit of another member is never run.
+    Thousands of such blocks among the regular members; every y checked."""
+    from test_gpu_formats import csr_from_lengths
+    rp, ci, va = csr_from_lengths([3, 300, 0] * 2000, 5000, 7)
+    m = rp.size - 1
+    f = hip.csr_to_format(rp, ci, va, m, 5000, ci.size, kind="csr_vec")
+    x = gen.make_x(5000, 77)
+    dx, dy = hip.DeviceArray.from_host(x), hip.DeviceArray(m)
+    y_ref = pyoracle.csr_naive(rp, ci, va, x)
+    b = hip.Batch(1)
+    b.set_xcd_ranges(ranges)
+    b.add(f, dx, dy)
+    for mb in members:
+        b.add(mb["fmt"], mb["dx"], mb["dy"])
+    b.finalize(use_graph=True)
+    for rep in range(3):
+        dy.upload(np.full(m, np.nan))
+        for mb in members:
+            mb["dy"].upload(np.full(mb["m"], np.nan))
+        b.run()
+        b.synchronize()
+        assert pyoracle.normwise_rel_err(dy.download(), y_ref) <= 1e-13
+        check(members)
+    b.destroy()
