@@ -188,7 +188,7 @@ def check_leg(a, x, ys, best_key):
 
 
 PROFILED = {  # the variant each profile summary's byte counters were taken on
-    ("spmv8m", "csr_vec"): {"block_nnz": 256},
+    ("spmv8m", "csr_vec"): {"block_nnz": 256, "xcd_remap": 1},
     ("spmv8m", "sell_c_s"): {"sigma": 256, "lanes_per_row": 4, "unroll": 4},
     ("spmv8m", "div_rf"): {"div_packet_vals": 1024, "div_threads": 64, "unroll": 8, "xcd_remap": 1},
 }

@@ -17,6 +17,8 @@
 //   CSRV_BLOCK_NNZ, CSRV_LANES, CSRV_SPLIT_NNZ, CSRV_NARROW_LOADS   CSR-vector row-block tuning
 //   SPMV_XCD_REMAP, SELL_UNROLL, SELL_PIPELINE, SELL_X_WINDOW, SELL_Y_MODE
 //   DIV_THREADS, DIV_LANES, CSRCV_NUM_PACKET_VALS, DIV_LAYOUT   DIV-RF codec / device layout
+//   DIV_ENCODER                      where the DIV-RF packets are encoded: 0 automatic
+//                                    (the device when possible), 1 host, 2 device
 //   DIV_PACKET_REORDERING=1          packet groups in median-column order (config.sh's variable)
 //   DIV_VC_TOLERANCE=t               lossy DIV-RF values, relative error <= t (config.sh's variable)
 
@@ -121,6 +123,7 @@ struct Matrix_Format *csr_to_format(INT_T *row_ptr, INT_T *col_ind, ValueTypeRef
 	p.narrow_loads = (int) env_long("CSRV_NARROW_LOADS", 0);
 	p.div_layout = (int) env_long("DIV_LAYOUT", 0);
 	p.div_reorder = (int) env_long("DIV_PACKET_REORDERING", 0);
+	p.div_encoder = (int) env_long("DIV_ENCODER", 0);
 	if (const char *tol = getenv("DIV_VC_TOLERANCE"))
 		p.div_tolerance = atof(tol);
 	p.div_packet_vals = env_long("CSRCV_NUM_PACKET_VALS", 0);
