@@ -253,9 +253,6 @@ def probe(hip, np):
     print(json.dumps({"probe": out}), flush=True)
 
 
-E2E_RANGES = dict(rows=(1e4, 4e6), avg=(2, 128), skew=(1, 1000), distributions=("gamma",))
-
-
 def run_e2e(args, gen, hip, np):
     """End-to-end streaming of the held-out workload (master seed 55: rows =
     cols log-uniform in [1e4, 4e6], avg 2..128, gamma rows, skew up to 1000;
@@ -268,17 +265,17 @@ def run_e2e(args, gen, hip, np):
     model = os.path.join(ROOT, "spmv-dare_amd", "spmvdare", "models", "selector.json")
     sel = tuning.Selector.load(model)
     count = args.e2e_count
-    params = tuning.sample_params(55, max(count * 2, count + 64), **E2E_RANGES)
+    params = tuning.sample_params(55, max(count * 2, count + 64), **tuning.STREAM_RANGES)
     draws = "first %d matrices" % count
     if args.e2e_draws:
         # a slice of the stream by draw index (every generatable matrix of
         # draws [a, b)), so that a long stream can be run in several parts
         a_, b_ = (int(v) for v in args.e2e_draws.split(":"))
-        params = tuning.sample_params(55, b_, **E2E_RANGES)[a_:b_]
+        params = tuning.sample_params(55, b_, **tuning.STREAM_RANGES)[a_:b_]
         count = b_ - a_
         draws = "every generatable matrix of draws [%d, %d)" % (a_, b_)
     r = stream.run_stream(params, count, max(1, args.e2e_chunk), sel, tuned=args.e2e_tuned,
-                          fallback_margin=args.fallback_margin, steps=args.e2e_steps, batch_bnnz=args.batch_bnnz,
+                          fallback_spread=args.fallback_spread, steps=args.e2e_steps, batch_bnnz=args.batch_bnnz,
                           queues=args.queues, tile=args.tile, log=log, refine=args.refine)
     out = {
         "metric": "end-to-end batched FP64 SpMV matrices/s (selector-chosen formats), streaming",
@@ -292,7 +289,7 @@ def run_e2e(args, gen, hip, np):
                                  "matrices_per_s": round(r["plain_matrices_per_s"], 1)},
         "speedup_vs_plain_csr": round(r["speedup_vs_plain_csr"], 3),
         "max_normwise_err_vs_csr_naive": r["max_normwise_err_vs_csr_naive"],
-        "selector": {"fallback_margin": r["fallback_margin"], "fallbacks_to_exhaustive": r["fallbacks"],
+        "selector": {"fallback_spread": r["fallback_spread"], "fallbacks_to_exhaustive": r["fallbacks"],
                      "refine_top_k": r["refine"],
                      "regret_vs_exhaustive": r["regret"]},
         "wall_s": round(r["wall_s"], 1), "chosen_variants": r["chosen_variants"],
@@ -405,6 +402,49 @@ def batched_leg(args, gen, hip, np, count, steps, warmup):
     return out
 
 
+def run_member_times(args, gen, hip, np):
+    """--member-times: every member of the batched workload timed alone as a
+    plain binned-CSR launch (median of 10 after 3 warm-up launches, caches
+    warm), summarised by member size and by neighbours per nonzero (how many
+    of a row's nonzeros share cache lines of x). Prints one JSON line; it
+    shows which members a faster gather would help."""
+    from spmvdare import tuning
+    recs = []
+    for p in tuning.sample_params(21, args.count * 2):
+        if len(recs) == args.count:
+            break
+        try:
+            a = gen.generate(p, stats=False)
+        except gen.GenError:
+            continue
+        dm = tuning.DeviceCSR(a, p.seed + 1)
+        f = dm.convert(tuning.DEFAULT_CSR)
+        ms = float(np.median(f.time(dm.dx, dm.dy, 3, 10)))
+        f.destroy()
+        recs.append((a.nnz, alg_bytes(a.m, a.n, a.nnz), p.avg_num_neighbours, p.placement, ms))
+        del dm, a
+        if len(recs) % 256 == 0:
+            log("member times: %d/%d" % (len(recs), args.count))
+    nnz = np.array([r[0] for r in recs], dtype=np.float64)
+    byt = np.array([r[1] for r in recs], dtype=np.float64)
+    nb = np.array([r[2] for r in recs])
+    rnd = np.array([r[3] == "random" for r in recs])
+    ms = np.array([r[4] for r in recs])
+
+    def cls(mask):
+        return {"members": int(mask.sum()), "ms": round(float(ms[mask].sum()), 3),
+                "gbps": round(float(byt[mask].sum() / (ms[mask].sum() * 1e-3) / 1e9), 1) if mask.any() else None}
+
+    out = {"workload": "%d generated matrices, master seed 21, each a plain binned-CSR launch" % len(recs),
+           "sum_ms": round(float(ms.sum()), 3), "gbps": round(float(byt.sum() / (ms.sum() * 1e-3) / 1e9), 1),
+           "by_nnz": {"%g..%g" % (lo, hi): cls((nnz >= lo) & (nnz < hi))
+                      for lo, hi in ((0, 1e5), (1e5, 1e6), (1e6, 1e7), (1e7, 1e9))},
+           "nnz_ge_1e6_by_neighbours": {"%g..%g" % (lo, hi): cls((nnz >= 1e6) & (nb >= lo) & (nb < hi))
+                                        for lo, hi in ((0, 1), (1, 2), (2, 4), (4, 9))},
+           "by_placement": {"random": cls(rnd), "diagonal": cls(~rnd)}}
+    print(json.dumps(out), flush=True)
+
+
 def run_batched(args, gen, hip, np):
     """--batched: the batched leg alone, as its own JSON line (matrices/s)."""
     b = batched_leg(args, gen, hip, np, args.count, args.steps, args.warmup)
@@ -420,6 +460,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--probe", action="store_true", help="measure stream read / copy bandwidth only")
     ap.add_argument("--batched", action="store_true", help="batched workload only (its own JSON line)")
+    ap.add_argument("--member-times", action="store_true",
+                    help="each member of the batched workload timed alone as plain CSR, by member class")
     ap.add_argument("--no-batched", action="store_true", help="leave the batched leg out of the default run")
     ap.add_argument("--count", type=int, default=2048, help="matrices of the batched workload")
     ap.add_argument("--tile", type=int, default=0,
@@ -433,8 +475,9 @@ def main():
     ap.add_argument("--e2e-count", type=int, default=1000, help="matrices of the stream")
     ap.add_argument("--e2e-chunk", type=int, default=16, help="matrices per chunk of the stream")
     ap.add_argument("--e2e-tuned", type=int, default=32, help="matrices of the stream also tuned exhaustively")
-    ap.add_argument("--fallback-margin", type=float, default=0.05,
-                    help="selector falls back to exhaustive tuning when its two best predictions are this close")
+    ap.add_argument("--fallback-spread", type=float, default=0.3,
+                    help="selector falls back to exhaustive tuning when the timed ratios of its best predictions "
+                         "differ from the predicted ratios by more than this")
     ap.add_argument("--e2e-steps", type=int, default=10, help="timed launches per chunk")
     ap.add_argument("--e2e-draws", default=None, help="A:B -- every generatable matrix of stream draws [A, B)")
     ap.add_argument("--refine", type=int, default=3, help="time the k best predicted variants, keep the fastest")
@@ -468,6 +511,9 @@ def main():
         return
     if args.e2e:
         run_e2e(args, gen, hip, np)
+        return
+    if args.member_times:
+        run_member_times(args, gen, hip, np)
         return
     line, desc = WORKLOADS[args.workload]
     p = gen.GenParams.parse(line)

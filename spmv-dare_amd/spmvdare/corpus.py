@@ -13,8 +13,14 @@ merge (columns in variant-list order):
 Matrices are sampled from the batched workload's parameter ranges (rows = cols
 log-uniform in [1e3, 2e5], avg 4..64, std 0.25..2 x avg, normal / gamma,
 random / diagonal, bw 0.01..1, skew 1..100, neighbours 0..8, cross-row
-similarity 0..0.9). A parameter set the generator rejects (a cross-row copy
-collision, where the reference exits) is replaced by the next draw.
+similarity 0..0.9), or with `--ranges stream` from the streaming workload's
+(rows log-uniform in [1e4, 4e6], avg 2..128, gamma rows, skew 1..1000, the
+rest as above):
+
+    python -m spmvdare.corpus --ranges stream --seed 56 --budget 900 --out stream_train.npz
+
+A parameter set the generator rejects (a cross-row copy collision, where the
+reference exits) is replaced by the next draw.
 """
 import argparse
 import sys
@@ -26,12 +32,13 @@ from . import gen, hip, tuning
 
 
 def build(count, seed, warmup=2, iters=8, log=None, ranges=None, budget_s=None, start=0, stop=None,
-          draws=None, variants=None):
+          draws=None, variants=None, checkpoint=None):
     """Time `count` matrices drawn from sample_params(seed), beginning at draw
     `start`, or (stop given) every matrix of the draws [start, stop), so that
     a corpus can be built in several sessions and concatenated; or (draws
     given) exactly those draws. `variants`: indices into tuning.VARIANTS to
-    time (default all)."""
+    time (default all). `checkpoint(arrays)` is called with the corpus so far
+    every 32 matrices; `log` gets a progress line every 8."""
     if draws is not None:
         draws = [int(d) for d in draws]
         params = tuning.sample_params(seed, max(draws) + 1 if draws else 0, **(ranges or {}))
@@ -68,8 +75,11 @@ def build(count, seed, warmup=2, iters=8, log=None, ranges=None, budget_s=None, 
         pidx.append(i)
         del dm
         k += 1
-        if log and (k % 32 == 0 or k == count):
+        if log and (k % 8 == 0 or k == count):
             log("corpus %d/%d matrices, %.0f s" % (k, count, time.time() - t0))
+        if k % 32 == 0 or k == count:
+            if checkpoint:
+                checkpoint((np.array(feats), np.array(times), np.array(lines), np.array(nnzs), np.array(pidx)))
     return np.array(feats), np.array(times), np.array(lines), np.array(nnzs), np.array(pidx)
 
 
@@ -108,6 +118,8 @@ def main():
                     help="time every matrix of the draws [start, stop) (instead of --count matrices)")
     ap.add_argument("--draws-from", default=None, help="time the draws of this corpus (same --seed)")
     ap.add_argument("--variants", default=None, help="comma-separated variant indices (default all)")
+    ap.add_argument("--ranges", default="batched", choices=sorted(tuning.WORKLOAD_RANGES),
+                    help="parameter ranges of the draws: the batched or the streaming workload's")
     args = ap.parse_args()
     draws = np.load(args.draws_from)["draw"] if args.draws_from else None
     vidx = [int(v) for v in args.variants.split(",")] if args.variants else None
@@ -116,11 +128,18 @@ def main():
     def log(s):
         print(s, file=sys.stderr, flush=True)
 
-    X, T, L, N, I = build(args.count, args.seed, iters=args.iters, log=log, budget_s=args.budget,
-                          start=args.start, stop=args.stop, draws=draws, variants=vidx)
     names = [tuning.variant_name(i) for i in (vidx if vidx is not None else range(len(tuning.VARIANTS)))]
-    np.savez_compressed(args.out, features=X, times=T, lines=L, nnz=N, draw=I, variants=np.array(names))
-    log("wrote %s: %d matrices x %d variants" % (args.out, T.shape[0], T.shape[1]))
+
+    def save(arrays):
+        X, T, L, N, I = arrays
+        np.savez_compressed(args.out, features=X, times=T, lines=L, nnz=N, draw=I, variants=np.array(names),
+                            ranges=np.array(args.ranges), seed=np.array(args.seed))
+
+    arrays = build(args.count, args.seed, iters=args.iters, log=log, budget_s=args.budget,
+                   start=args.start, stop=args.stop, draws=draws, variants=vidx,
+                   ranges=tuning.WORKLOAD_RANGES[args.ranges], checkpoint=save)
+    save(arrays)
+    log("wrote %s: %d matrices x %d variants" % (args.out, arrays[1].shape[0], len(names)))
 
 
 if __name__ == "__main__":

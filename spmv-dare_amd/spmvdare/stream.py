@@ -7,9 +7,10 @@ interpreter lock) while the GPU takes chunk i: each matrix is uploaded, its
 structure statistics are computed on the device, the learned selector
 predicts every variant's slowdown, and the matrix is converted to the chosen
 variant: the `refine` best predicted variants are timed and the fastest
-is kept, and when the selector is unsure (its two best predictions lie
-within `fallback_margin` of each other) the choice falls back to exhaustive
-autotuning over every variant the batched launch can run. The chunk then
+is kept, and when those timings contradict the predictions (their ratios
+differ from the predicted ones by more than `fallback_spread`: the selector
+is off on this matrix) the choice falls back to exhaustive autotuning over
+every variant the batched launch can run. The chunk then
 runs as one persistent batched launch replayed from a hipGraph, and is
 compared on the same GPU with plain per-matrix CSR-vector launches. Every
 member's y is checked against the device csr_naive of its matrix (a
@@ -38,37 +39,56 @@ def _rel(y, yr):
     return float(np.linalg.norm(y - yr) / nrm) if nrm > 0 else float(np.linalg.norm(y))
 
 
-def choose(sel, dm, p, allowed, fallback_margin, tune, refine=3, warmup=1, iters=5):
+def consistency_spread(pred, times):
+    """How far timings contradict the selector's predictions. The selector
+    predicts log(t_v / t_best) for every variant v, so each timed variant
+    gives an estimate log(t_v) - pred_v of log t_best; when the predictions
+    are right these agree. Returns the range of those estimates (natural
+    log): 0 when the timed ratios are exactly the predicted ones."""
+    e = np.log(np.asarray(times, dtype=np.float64)) - np.asarray(pred, dtype=np.float64)
+    return float(e.max() - e.min())
+
+
+def choose(sel, dm, p, allowed, fallback_spread, tune, refine=3, warmup=1, iters=5):
     """The variant a matrix runs as, and what was learned choosing it.
-    Returns (variant, info): info["fallback"] when the selector's two best
-    predictions were within fallback_margin (then the choice is the
-    exhaustive tuner's); otherwise the fastest of the `refine` best
-    predictions, timed (1: the top prediction untimed). info["times"]:
-    exhaustive timings over `allowed` (ms, NaN elsewhere) when tuned."""
+
+    The `refine` best predictions are timed and the fastest is kept. When
+    their timings disagree with the predictions by more than fallback_spread
+    (consistency_spread above log1p(fallback_spread): the selector is off on
+    this matrix, e.g. outside what it was trained on), the choice falls back
+    to exhaustive autotuning over `allowed`. With refine = 1 the top
+    prediction is kept untimed and nothing falls back. With `tune` every
+    allowed variant is timed first (to judge the regret) and the policy reads
+    its timings from that run. Returns (variant, info): info["fallback"],
+    info["spread"], info["top1"], and info["times"] (ms over `allowed`, NaN
+    elsewhere) when exhaustive timings were taken."""
     feats = tuning.features(p, dm.stats())
     pred = sel.predict(feats)[0].astype(np.float64)
     pred[~allowed] = np.inf
     order = np.argsort(pred)
-    margin = float(pred[order[1]] - pred[order[0]])
-    fallback = margin < math.log1p(fallback_margin)
-    info = {"top1": int(order[0]), "margin": margin, "fallback": bool(fallback)}
+    info = {"top1": int(order[0]), "fallback": False, "spread": 0.0}
     t = None
-    if fallback or tune:
-        idx = np.flatnonzero(allowed)
+    idx = np.flatnonzero(allowed)
+    if tune:
         t = np.full(len(tuning.VARIANTS), np.nan)
         t[idx] = tuning.autotune(dm, variants=idx, warmup=warmup, iters=iters)
         info["times"] = t
-        if fallback:
-            return int(np.nanargmin(t)), info
     top = [int(i) for i in order[:max(1, refine)] if np.isfinite(pred[i])]
     if len(top) == 1:
         return top[0], info
-    # the fastest of the best predictions (timings from the exhaustive run when there is one)
     tt = t[top] if t is not None else tuning.autotune(dm, variants=top, warmup=warmup, iters=iters)
+    info["spread"] = consistency_spread(pred[top], tt)
+    if info["spread"] > math.log1p(fallback_spread):
+        info["fallback"] = True
+        if t is None:
+            t = np.full(len(tuning.VARIANTS), np.nan)
+            t[idx] = tuning.autotune(dm, variants=idx, warmup=warmup, iters=iters)
+            info["times"] = t
+        return int(np.nanargmin(t)), info
     return top[int(np.argmin(tt))], info
 
 
-def run_stream(params, count, chunk, selector, tuned=0, fallback_margin=0.05, steps=10, batch_bnnz=256,
+def run_stream(params, count, chunk, selector, tuned=0, fallback_spread=0.3, steps=10, batch_bnnz=256,
                queues=256, tile=0, log=None, keep_y=False, solo_nnz=0, refine=3):
     """Process the first `count` generatable matrices of `params` (a list of
     gen.GenParams) as described in the module docstring. Returns a dict of
@@ -103,7 +123,7 @@ def run_stream(params, count, chunk, selector, tuned=0, fallback_margin=0.05, st
         mats = []
         for p, a in pending:
             dm = tuning.DeviceCSR(a, p.seed + 1)
-            v, info = choose(selector, dm, p, allowed, fallback_margin, done + len(mats) < tuned, refine=refine)
+            v, info = choose(selector, dm, p, allowed, fallback_spread, done + len(mats) < tuned, refine=refine)
             fallbacks += info["fallback"]
             if "times" in info and done + len(mats) < tuned:
                 t = info["times"]
@@ -164,7 +184,7 @@ def run_stream(params, count, chunk, selector, tuned=0, fallback_margin=0.05, st
         "speedup_vs_plain_csr": t_p / t_b if t_b else None,
         "max_normwise_err_vs_csr_naive": worst,
         "max_normwise_err_vs_csr_naive_plain_csr": worst_plain,
-        "fallback_margin": fallback_margin, "fallbacks": fallbacks, "refine": refine,
+        "fallback_spread": fallback_spread, "fallbacks": fallbacks, "refine": refine,
         "regret": {"matrices": len(regrets),
                    "top1_mean": float(r[:, 0].mean()) if len(r) else None,
                    "top1_max": float(r[:, 0].max()) if len(r) else None,

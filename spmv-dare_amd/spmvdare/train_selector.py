@@ -1,20 +1,35 @@
-"""Train the format selector on a corpus and judge it on a held-out corpus.
+"""Train the format selector on corpora and judge it on held-out corpora.
 
-    python -m spmvdare.train_selector --train train.npz --test test.npz --out selector.json
+    python -m spmvdare.train_selector --train train.npz [more.npz ...] \
+        --test test.npz [more_test.npz ...] --out selector.json
 
-Reports, on the held-out matrices, the regret t_chosen / t_best - 1 against
-exhaustive autotuning for: the MLP's top choice, the MLP's top-3 refined by
-timing them (the fallback), the single variant best on average over the
-training set (the method without the learned part), and the default
-binned-CSR variant.
+The training corpora are concatenated. Reports, on each held-out corpus
+separately, the regret t_chosen / t_best - 1 against exhaustive autotuning
+for: the MLP's top choice, the MLP's top-3 refined by timing them (the
+fallback), the single variant best on average over the training set (the
+method without the learned part), and the default binned-CSR variant.
 """
 import argparse
 import json
+import os
 import sys
 
 import numpy as np
 
 from . import hip, tuning
+
+
+def load_corpora(paths):
+    """Features and timings of several corpora (same variant list), stacked."""
+    names = [tuning.variant_name(i) for i in range(len(tuning.VARIANTS))]
+    X, T = [], []
+    for path in paths:
+        c = np.load(path)
+        if list(c["variants"]) != names:
+            raise SystemExit("%s was timed on another variant list" % path)
+        X.append(c["features"])
+        T.append(c["times"])
+    return np.concatenate(X), np.concatenate(T)
 
 
 def regrets(times, choice):
@@ -48,22 +63,23 @@ def evaluate(sel, Xte, Tte, Ttr, k=3):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train", required=True)
-    ap.add_argument("--test", required=True)
+    ap.add_argument("--train", nargs="+", required=True)
+    ap.add_argument("--test", nargs="+", required=True)
     ap.add_argument("--out", required=True)
     ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--lr", type=float, default=2e-3)
     args = ap.parse_args()
     hip.init(0)
-    tr = np.load(args.train)
-    te = np.load(args.test)
+    X, T = load_corpora(args.train)
     sel = tuning.Selector()
-    losses = sel.fit(tr["features"], tr["times"], epochs=args.epochs, lr=args.lr)
+    losses = sel.fit(X, T, epochs=args.epochs, lr=args.lr)
     print("train loss: first %.4f last %.4f" % (losses[0], losses[-1]), file=sys.stderr, flush=True)
-    res = evaluate(sel, te["features"], te["times"], tr["times"])
-    res["train_matrices"] = int(len(tr["times"]))
-    res["test_matrices"] = int(len(te["times"]))
-    res["final_train_loss"] = float(losses[-1])
+    res = {"train_matrices": int(len(T)), "final_train_loss": float(losses[-1]), "test": {}}
+    for path in args.test:
+        te = np.load(path)
+        r = evaluate(sel, te["features"], te["times"], T)
+        r["test_matrices"] = int(len(te["times"]))
+        res["test"][os.path.basename(path)] = r
     sel.save(args.out)
     print(json.dumps(res), flush=True)
 

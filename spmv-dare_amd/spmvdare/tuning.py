@@ -109,6 +109,13 @@ def sample_params(master_seed, count, rows=(1e3, 2e5), avg=(4, 64), std_ratio=(0
     return out
 
 
+# The streaming workload's ranges (rows = cols log-uniform in [1e4, 4e6], avg
+# 2..128, gamma rows, skew up to 1000; the other descriptors as above).
+STREAM_RANGES = dict(rows=(1e4, 4e6), avg=(2, 128), skew=(1, 1000), distributions=("gamma",))
+# sample_params keyword ranges by workload name
+WORKLOAD_RANGES = {"batched": {}, "stream": STREAM_RANGES}
+
+
 # ------------------------------------------------------------------ features
 
 FEATURE_NAMES = ["log_rows", "log_cols", "log_avg", "std_over_avg", "gamma", "diagonal", "bw", "log1p_skew",

@@ -74,6 +74,23 @@ def test_committed_selector_regret():
     assert res["mlp_top3_refined"]["mean"] <= 0.05
 
 
+def test_committed_selector_regret_on_stream():
+    """The committed model on the held-out streaming corpus (the first 128
+    matrices of the master-seed-55 stream, rows up to 4e6, never trained
+    on; the training corpora are the batched workload's and seed 56 of the
+    stream's ranges): its top choice beats the static best variant and its
+    top-3 refined by timing stays within the 5 % regret target."""
+    trs = [np.load(os.path.join(MODELS, f)) for f in ("corpus_train.npz", "stream_train.npz")]
+    te = np.load(os.path.join(MODELS, "stream_test.npz"))
+    assert str(te["ranges"]) == "stream" and int(te["seed"]) == 55
+    assert {int(t["seed"]) for t in trs if "seed" in t.files} == {56}
+    sel = tuning.Selector.load(os.path.join(MODELS, "selector.json"))
+    res = evaluate(sel, te["features"], te["times"], np.concatenate([t["times"] for t in trs]))
+    print(json.dumps(res, indent=1))
+    assert res["mlp_top1"]["mean"] < res["static_best_variant"]["mean"]
+    assert res["mlp_top3_refined"]["mean"] <= 0.05
+
+
 def test_autotune_cache_reuses_timings(tmp_path):
     """Timings of a matrix already tuned on this device come from the cache;
     only variants not timed before are run."""

@@ -14,7 +14,7 @@ import pyoracle
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 MODEL = os.path.join(ROOT, "spmv-dare_amd", "spmvdare", "models", "selector.json")
-RANGES = dict(rows=(1e4, 4e6), avg=(2, 128), skew=(1, 1000), distributions=("gamma",))
+RANGES = tuning.STREAM_RANGES
 
 
 @pytest.fixture(scope="module")
@@ -25,7 +25,7 @@ def selector():
 
 def test_stream_first_16_matrices(selector):
     params = tuning.sample_params(55, 64, **RANGES)
-    r = stream.run_stream(params, 16, 8, selector, tuned=2, fallback_margin=0.05, steps=3, keep_y=True,
+    r = stream.run_stream(params, 16, 8, selector, tuned=2, fallback_spread=0.3, steps=3, keep_y=True,
                           log=lambda s: print(s, flush=True))
     print({k: v for k, v in r.items() if k != "y"}, flush=True)
     assert r["matrices"] == 16 and len(r["y"]) == 16
@@ -42,10 +42,10 @@ def test_stream_first_16_matrices(selector):
 
 
 def test_stream_fallback_is_exhaustive(selector):
-    """With a margin no prediction pair can beat, every choice falls back to
-    exhaustive tuning, so the policy's regret against that same tuning is 0."""
+    """With a negative spread limit every choice falls back to exhaustive
+    tuning, so the policy's regret against that same tuning is 0."""
     params = tuning.sample_params(21, 8)   # small matrices of the batched workload's ranges
-    r = stream.run_stream(params, 3, 3, selector, tuned=3, fallback_margin=1e9, steps=2)
+    r = stream.run_stream(params, 3, 3, selector, tuned=3, fallback_spread=-0.5, steps=2)
     assert r["fallbacks"] == 3
     assert r["regret"]["policy_max"] == 0.0
     assert r["max_normwise_err_vs_csr_naive"] <= 1e-13

@@ -17,7 +17,7 @@ def test_model_and_corpora_use_the_current_variants():
     d = json.load(open(os.path.join(MODELS, "selector.json")))
     assert d["variants"] == NAMES
     assert d["dims"] == [14, 64, 64, len(NAMES)]
-    for f in ("corpus_train.npz", "corpus_test.npz"):
+    for f in ("corpus_train.npz", "corpus_test.npz", "stream_train.npz", "stream_test.npz"):
         c = np.load(os.path.join(MODELS, f), allow_pickle=False)
         assert list(c["variants"]) == NAMES
         assert c["times"].shape[1] == len(NAMES)
@@ -30,3 +30,55 @@ def test_load_rejects_another_variant_list(tmp_path):
     p.write_text(json.dumps(d))
     with pytest.raises(ValueError):
         tuning.Selector.load(str(p))
+
+
+def test_stream_ranges_draws():
+    """The streaming workload's draws: rows = cols log-uniform in [1e4, 4e6],
+    avg 2..128, gamma rows only, skew up to 1000; the batched workload's
+    draws stay within its own ranges."""
+    ps = tuning.sample_params(56, 400, **tuning.WORKLOAD_RANGES["stream"])
+    rows = np.array([p.nr_rows for p in ps])
+    assert rows.min() >= 1e4 and rows.max() <= 4e6
+    assert np.median(rows) > 2e5  # log-uniform over [1e4, 4e6]: median 2e5
+    assert {p.distribution for p in ps} == {"gamma"}
+    assert all(2 <= p.avg_nnz_per_row <= 128 and 1 <= p.skew <= 1000 for p in ps)
+    assert all(p.nr_rows == p.nr_cols for p in ps)
+    assert max(p.skew for p in ps) > 100
+    qs = tuning.sample_params(56, 400, **tuning.WORKLOAD_RANGES["batched"])
+    assert all(1e3 <= q.nr_rows <= 2e5 and q.skew <= 100 for q in qs)
+
+
+def test_load_corpora_stacks_and_checks_variants(tmp_path):
+    from spmvdare.train_selector import load_corpora
+    rng = np.random.default_rng(0)
+    paths = []
+    for k, rows in enumerate((3, 5)):
+        p = tmp_path / ("c%d.npz" % k)
+        np.savez_compressed(p, features=rng.random((rows, 14)), times=rng.random((rows, len(NAMES))) + 0.1,
+                            variants=np.array(NAMES))
+        paths.append(str(p))
+    X, T = load_corpora(paths)
+    assert X.shape == (8, 14) and T.shape == (8, len(NAMES))
+    np.testing.assert_array_equal(T[3:], np.load(paths[1])["times"])
+    bad = tmp_path / "bad.npz"
+    np.savez_compressed(bad, features=np.zeros((1, 14)), times=np.ones((1, len(NAMES))),
+                        variants=np.array(list(reversed(NAMES))))
+    with pytest.raises(SystemExit):
+        load_corpora([paths[0], str(bad)])
+
+
+def test_consistency_spread():
+    """The stream's fallback trigger: timings whose ratios are the predicted
+    ones give spread 0 whatever the absolute time; one timing twice as slow
+    as predicted relative to the others gives log 2."""
+    from spmvdare.stream import consistency_spread
+    pred = np.array([0.0, 0.1, 0.4])
+    t = 3.0 * np.exp(pred)
+    assert abs(consistency_spread(pred, t)) < 1e-12
+    assert abs(consistency_spread(pred, t * 7.5)) < 1e-12
+    t2 = t.copy()
+    t2[2] *= 2.0
+    assert abs(consistency_spread(pred, t2) - np.log(2.0)) < 1e-12
+    t3 = t.copy()
+    t3[0] *= 1.25  # the top prediction slower than predicted
+    assert abs(consistency_spread(pred, t3) - np.log(1.25)) < 1e-12
