@@ -304,7 +304,9 @@ def batched_leg(args, gen, hip, np, count, steps, warmup):
     """Batched workload: `count` generated matrices of uneven sizes (rows =
     cols log-uniform in [1e3, 2e5], avg 4..64, ...; master seed 21), each with
     its own x, computed by one persistent work-queue launch replayed from a
-    hipGraph, with every matrix's format chosen by the learned selector;
+    hipGraph, with every matrix's format chosen by the learned selector (a
+    DIV-RF choice kept only when predicted `--batch-div-margin` faster than
+    the best other variant: DIV members run in a second persistent kernel);
     compared on the same GPU with plain per-matrix binned-CSR launches (one
     launch per matrix, back to back) and with the same persistent launch over
     all-CSR members. Every member's batched y is checked against the device
@@ -314,6 +316,8 @@ def batched_leg(args, gen, hip, np, count, steps, warmup):
     model = os.path.join(ROOT, "spmv-dare_amd", "spmvdare", "models", "selector.json")
     sel = tuning.Selector.load(model) if os.path.exists(model) and not args.no_selector else None
     fixed = tuning.VARIANTS.index(("sell_c_s", {"sigma": 256, "lanes_per_row": 4, "unroll": 4}))
+    if args.batch_fixed is not None:
+        fixed, sel = args.batch_fixed, None
     csr = tuning.DEFAULT_CSR
     allowed = tuning.batch_allowed()
     params = tuning.sample_params(21, count * 2)
@@ -328,8 +332,8 @@ def batched_leg(args, gen, hip, np, count, steps, warmup):
             continue
         dm = tuning.DeviceCSR(a, p.seed + 1)
         if sel is not None:
-            v = tuning.batch_variant(int(sel.select(tuning.features(p, dm.stats()), allowed=allowed)[0, 0]),
-                                     args.batch_bnnz)
+            v = tuning.batch_choice(sel.predict(tuning.features(p, dm.stats()))[0], allowed, args.batch_bnnz,
+                                    args.batch_div_margin)
         else:
             v = fixed
         chosen.append(v)
@@ -351,14 +355,28 @@ def batched_leg(args, gen, hip, np, count, steps, warmup):
     for dm, f_sel, f_csr, dy_csr in members:
         batch.add(f_sel, dm.dx, dm.dy)
         base.add(f_csr, dm.dx, dy_csr)
-    batch.finalize(use_graph=True)
+    batch.finalize(use_graph=not args.no_graph)
     base.finalize(use_graph=False)
     hip.synchronize()
-    ms_batch = batch.time(warmup, steps)
-    ms_plain = base.time(max(1, warmup // 2), max(3, steps // 4), per_matrix=True)
-    # the same persistent launch with every member as plain binned CSR: the
-    # batching gain without the selector
-    ms_base = base.time(max(1, warmup // 2), max(3, steps // 4))
+    # The three legs are timed in interleaved rounds (each round: an untimed
+    # iteration, then the timed ones, of every leg in turn; the plain
+    # launches first, then the two persistent launches in alternating
+    # order), so that they run at the same clock and cache state: timed one
+    # after the other, the same launch ran about 2 % slower first than last.
+    # The third leg is the same persistent launch with every member as plain
+    # binned CSR: the batching gain without the selector.
+    batch.time(warmup, 1)
+    base.time(1, 1, per_matrix=True)
+    rounds = 4
+    ms_batch, ms_plain, ms_base = [], [], []
+    for r in range(rounds):
+        ms_plain += list(base.time(1, max(1, steps // (4 * rounds)), per_matrix=True))
+        for leg in ((0, 1) if r % 2 == 0 else (1, 0)):
+            if leg == 0:
+                ms_batch += list(batch.time(1, max(1, steps // rounds)))
+            else:
+                ms_base += list(base.time(1, max(1, steps // rounds)))
+    ms_batch, ms_plain, ms_base = np.array(ms_batch), np.array(ms_plain), np.array(ms_base)
     batch.run()
     batch.synchronize()
     # every member's batched y against its csr_naive y on the device
@@ -391,7 +409,8 @@ def batched_leg(args, gen, hip, np, count, steps, warmup):
         "max_normwise_err_vs_csr_naive": worst,
         "max_normwise_err_vs_csr_naive_plain_csr": worst_csr,
         "prep_s": round(prep_s, 1), "check_s": round(check_s, 1),
-        "batch_min_block_nnz": args.batch_bnnz,
+        "batch_min_block_nnz": args.batch_bnnz, "batch_div_margin": args.batch_div_margin,
+        "hip_graph": not args.no_graph,
         "chosen_variants": {tuning.variant_name(i): int(c) for i, c in enumerate(hist) if c},
     }
     log("batched %.3f ms (%.0f matrices/s, %.0f GB/s) vs plain per-matrix CSR %.3f ms: x%.2f; "
@@ -467,8 +486,15 @@ def main():
     ap.add_argument("--tile", type=int, default=0,
                     help="batched launch: consecutive units per ticket (0 = the library default)")
     ap.add_argument("--no-selector", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched launch: launch the persistent kernels directly instead of replaying a hipGraph")
+    ap.add_argument("--batch-fixed", type=int, default=None,
+                    help="batched launch: every member as this variant (index into the variant list), no selector")
     ap.add_argument("--batch-bnnz", type=int, default=256,
                     help="batched launch: CSR choices with smaller row blocks run with this block size (0 = as chosen)")
+    ap.add_argument("--batch-div-margin", type=float, default=0.1,
+                    help="batched launch: a DIV-RF choice is kept only when predicted this much faster than the "
+                         "best other variant (DIV members run in a second persistent kernel)")
     ap.add_argument("--queues", type=int, default=256,
                     help="batched launch: ticket counters the units are dealt over (a power of two)")
     ap.add_argument("--e2e", action="store_true", help="end-to-end streaming sample of the held-out workload")

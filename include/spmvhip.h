@@ -250,8 +250,11 @@ int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d
 int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph);
 int spmvhip_batch_run(spmvhip_batch *b);          /* asynchronous on the batch's stream */
 int spmvhip_batch_synchronize(spmvhip_batch *b);
-/* ms[i]: time of iteration i of the batched launch (per_matrix = 0) or of
- * one standalone spmv launch per member, back to back (per_matrix = 1). */
+/* ms[i]: GPU time of iteration i of the batched launch (per_matrix = 0) or
+ * of one standalone spmv launch per member, back to back (per_matrix = 1).
+ * After `warmup` untimed iterations, the `iters` timed ones are submitted
+ * back to back with an event between consecutive iterations and one wait at
+ * the end, so the host's submission overlaps the GPU's work. */
 int spmvhip_batch_time(spmvhip_batch *b, int per_matrix, int warmup, int iters, float *ms);
 /* nitems: units of the batch (row blocks, segments, slices); grid: workgroups of the launch. */
 int spmvhip_batch_info(const spmvhip_batch *b, long *nmats, long *nitems, int *grid, long *total_nnz);

@@ -621,30 +621,41 @@ int spmvhip_batch_time(spmvhip_batch *b, int per_matrix, int warmup, int iters, 
 		}
 		return SPMVHIP_OK;
 	};
+	// Iterations run back to back: an event between consecutive iterations
+	// and one wait at the end. The warm-up iterations are still running when
+	// the first event is recorded, so the host's submission of each
+	// iteration (a graph launch costs the host far more than a kernel
+	// launch) overlaps the GPU's work, and ms[i] is the GPU time of
+	// iteration i, as in a loop that replays the launch.
 	for (int i = 0; i < warmup; i++)
 		SPMVHIP_TRY(once());
-	hipEvent_t e0, e1;
-	SPMVHIP_CHECK(hipEventCreate(&e0));
-	SPMVHIP_CHECK(hipEventCreate(&e1));
+	std::vector<hipEvent_t> ev((size_t) iters + 1, nullptr);
 	int rc = SPMVHIP_OK;
-	for (int i = 0; i < iters && rc == SPMVHIP_OK; i++) {
-		hipError_t he = hipEventRecord(e0, b->stream);
-		if (he == hipSuccess)
-			rc = once();
-		if (he == hipSuccess && rc == SPMVHIP_OK)
-			he = hipEventRecord(e1, b->stream);
-		if (he == hipSuccess && rc == SPMVHIP_OK)
-			he = hipEventSynchronize(e1);
+	hipError_t he = hipSuccess;
+	for (size_t i = 0; i < ev.size() && he == hipSuccess; i++)
+		he = hipEventCreate(&ev[i]);
+	if (he == hipSuccess)
+		he = hipEventRecord(ev[0], b->stream);
+	for (int i = 0; i < iters && he == hipSuccess && rc == SPMVHIP_OK; i++) {
+		rc = once();
+		if (rc == SPMVHIP_OK)
+			he = hipEventRecord(ev[(size_t) i + 1], b->stream);
+	}
+	if (he == hipSuccess && rc == SPMVHIP_OK)
+		he = hipEventSynchronize(ev[(size_t) iters]);
+	for (int i = 0; i < iters && he == hipSuccess && rc == SPMVHIP_OK; i++) {
 		float t = 0;
-		if (he == hipSuccess && rc == SPMVHIP_OK)
-			he = hipEventElapsedTime(&t, e0, e1);
-		if (he != hipSuccess && rc == SPMVHIP_OK)
-			rc = set_error(SPMVHIP_EHIP, "batch timing: %s", hipGetErrorString(he));
+		he = hipEventElapsedTime(&t, ev[(size_t) i], ev[(size_t) i + 1]);
 		if (ms)
 			ms[i] = t;
 	}
-	(void) hipEventDestroy(e0);
-	(void) hipEventDestroy(e1);
+	if (he != hipSuccess && rc == SPMVHIP_OK)
+		rc = set_error(SPMVHIP_EHIP, "batch timing: %s", hipGetErrorString(he));
+	if (rc != SPMVHIP_OK)
+		(void) hipStreamSynchronize(b->stream); // nothing in flight references the events below
+	for (hipEvent_t e : ev)
+		if (e)
+			(void) hipEventDestroy(e);
 	return rc;
 }
 

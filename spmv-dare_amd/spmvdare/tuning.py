@@ -165,16 +165,44 @@ def batch_variant(i, min_block_nnz=256):
     small block only adds per-unit overhead (descriptor loads, a reduction)
     to a wave's sequential stream, so a CSR choice with fewer than
     `min_block_nnz` nonzeros per block is raised to that block size with the
-    same lanes-per-row setting. Other choices are kept. 0 keeps every choice.
+    same lanes-per-row setting (0 keeps every block size), and a CSR choice
+    with narrow loads runs with paired loads. Other choices are kept.
     """
     kind, kw = VARIANTS[i]
-    if kind != "csr_vec" or not min_block_nnz or kw.get("block_nnz", 256) >= min_block_nnz:
+    if kind != "csr_vec":
         return i
-    want = dict(kw, block_nnz=min_block_nnz)
+    want = dict(kw)
+    # one stored entry per lane and load helps a small matrix alone (more
+    # waves in flight); in a batch the paired loads' fewer instructions win
+    # (every member as either, 2,048 matrices: 9.12 against 8.96 ms)
+    want.pop("narrow_loads", None)
+    if min_block_nnz and kw.get("block_nnz", 256) < min_block_nnz:
+        want["block_nnz"] = min_block_nnz
+    if want == kw:
+        return i
     for j, (k2, kw2) in enumerate(VARIANTS):
         if k2 == kind and kw2 == want:
             return j
     return i
+
+
+def batch_choice(pred, allowed, min_block_nnz=256, div_margin=0.1):
+    """The variant a matrix runs as inside a persistent batched launch, from
+    the selector's predicted slowdowns `pred` (log t_v / t_best, one row).
+
+    DIV-RF members run in a second persistent kernel after the one that
+    takes every other member, so a DIV member adds that kernel's ramp and
+    tail to the launch; a DIV choice is kept only when it is predicted at
+    least `div_margin` faster than the best other allowed variant (else that
+    variant runs). The choice then goes through batch_variant."""
+    pred = np.where(np.asarray(allowed, dtype=bool), np.asarray(pred, dtype=np.float64), np.inf)
+    best = int(np.argmin(pred))
+    if VARIANTS[best][0] == "div_rf":
+        other = np.where(np.array([k != "div_rf" for k, _ in VARIANTS]), pred, np.inf)
+        alt = int(np.argmin(other))
+        if np.isfinite(other[alt]) and other[alt] - pred[best] < math.log1p(div_margin):
+            best = alt
+    return batch_variant(best, min_block_nnz)
 
 
 def time_variant(dm, i, warmup=2, iters=8):

@@ -82,3 +82,44 @@ def test_consistency_spread():
     t3 = t.copy()
     t3[0] *= 1.25  # the top prediction slower than predicted
     assert abs(consistency_spread(pred, t3) - np.log(1.25)) < 1e-12
+
+
+def test_batch_choice_div_margin():
+    """Inside a batched launch a DIV-RF choice must beat the best other
+    allowed variant by the margin; otherwise that variant runs (with the
+    batch's CSR block size)."""
+    allowed = tuning.batch_allowed()
+    div = tuning.VARIANTS.index(("div_rf", {"div_packet_vals": 1024, "div_threads": 1, "unroll": 8,
+                                            "xcd_remap": 1}))
+    csr128 = tuning.VARIANTS.index(("csr_vec", {"block_nnz": 128, "lanes_per_row": 0}))
+    csr256 = tuning.VARIANTS.index(("csr_vec", {"block_nnz": 256, "lanes_per_row": 0}))
+    csr512 = tuning.VARIANTS.index(("csr_vec", {"block_nnz": 512, "lanes_per_row": 0}))
+    pred = np.full(len(NAMES), 2.0)
+    pred[div], pred[csr128] = 0.0, np.log(1.05)        # DIV predicted 5 % faster: not enough
+    assert tuning.batch_choice(pred, allowed, 256, 0.1) == csr256
+    pred[csr128] = np.log(1.2)                           # 20 % faster: DIV kept
+    assert tuning.batch_choice(pred, allowed, 256, 0.1) == div
+    assert tuning.batch_choice(pred, allowed, 256, 0.5) == csr256
+    pred[:] = 2.0
+    pred[csr512], pred[csr256] = 0.0, 0.1                # block 512 is not allowed in a batch
+    assert tuning.batch_choice(pred, allowed, 256, 0.1) == csr256
+
+
+def test_batch_variant_mapping():
+    """Inside a batch: CSR blocks below the batch's block size are raised to
+    it with the same lanes per row, narrow loads become paired loads, other
+    choices are kept; 0 keeps every block size."""
+    V = tuning.VARIANTS
+    idx = lambda k, kw: V.index((k, kw))  # noqa: E731
+    assert tuning.batch_variant(idx("csr_vec", {"block_nnz": 64, "lanes_per_row": 1}), 256) == \
+        idx("csr_vec", {"block_nnz": 256, "lanes_per_row": 1})
+    assert tuning.batch_variant(idx("csr_vec", {"block_nnz": 128, "lanes_per_row": 4}), 256) == \
+        idx("csr_vec", {"block_nnz": 256, "lanes_per_row": 4})
+    narrow = idx("csr_vec", {"block_nnz": 256, "lanes_per_row": 0, "narrow_loads": 1})
+    assert tuning.batch_variant(narrow, 256) == idx("csr_vec", {"block_nnz": 256, "lanes_per_row": 0})
+    assert tuning.batch_variant(narrow, 0) == idx("csr_vec", {"block_nnz": 256, "lanes_per_row": 0})
+    small = idx("csr_vec", {"block_nnz": 64, "lanes_per_row": 0})
+    assert tuning.batch_variant(small, 0) == small
+    for k, kw in V:
+        if k != "csr_vec":
+            assert tuning.batch_variant(V.index((k, kw)), 256) == V.index((k, kw))
