@@ -118,7 +118,13 @@ def run_stream(params, count, chunk, selector, tuned=0, fallback_spread=0.3, ste
         want = min(chunk, count - done - len(pending))
         worker = None
         if want > 0:
-            worker = threading.Thread(target=lambda: box.update(m=produce(want)))
+            def fill(n=want, box=box):
+                try:
+                    box["m"] = produce(n)
+                except BaseException as e:  # handed to the main thread below
+                    box["error"] = e
+
+            worker = threading.Thread(target=fill)
             worker.start()
         mats = []
         for p, a in pending:
@@ -172,6 +178,8 @@ def run_stream(params, count, chunk, selector, tuned=0, fallback_spread=0.3, ste
                 done, count, time.perf_counter() - t_start, tot["ms_batched"], tot["ms_plain"], fallbacks))
         if worker is not None:
             worker.join()
+        if "error" in box:
+            raise RuntimeError("stream: host generation of the next chunk failed") from box["error"]
         pending = box.get("m", [])
     t_b, t_p = tot["ms_batched"], tot["ms_plain"]
     r = np.array(regrets) if regrets else np.zeros((0, 2))

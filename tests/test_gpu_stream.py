@@ -49,3 +49,22 @@ def test_stream_fallback_is_exhaustive(selector):
     assert r["fallbacks"] == 3
     assert r["regret"]["policy_max"] == 0.0
     assert r["max_normwise_err_vs_csr_naive"] <= 1e-13
+
+
+def test_stream_raises_when_host_generation_fails(selector, monkeypatch):
+    """A failure of the host thread that generates the next chunk reaches the
+    caller (the stream does not end early as if it were done)."""
+    params = tuning.sample_params(21, 8)   # small matrices of the batched workload's ranges
+    real = stream.gen.generate
+    calls = {"n": 0}
+
+    def failing(p, *a, **kw):
+        calls["n"] += 1
+        if calls["n"] > 2:  # the first chunk (2 matrices) generates, the next one fails
+            raise MemoryError("host generation failed")
+        return real(p, *a, **kw)
+
+    monkeypatch.setattr(stream.gen, "generate", failing)
+    with pytest.raises(RuntimeError) as e:
+        stream.run_stream(params, 4, 2, selector, steps=1)
+    assert isinstance(e.value.__cause__, MemoryError)
