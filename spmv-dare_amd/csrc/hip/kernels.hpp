@@ -156,6 +156,38 @@ __device__ __forceinline__ void block_rows_reduce(const double *p, int ra, int n
 	wave_lds_sync(); // the buffer is reused by the wave's next block
 }
 
+// One stream step of csr_block_at's paired layout: U halves of 128 entries
+// from the even index b (lane l takes the pairs at b + 2 l and, for U = 2,
+// b + 2 l + 128); lanes past the block re-read its last pair jl; products
+// of entries in [j0, j1) go to p[j - j0].
+template <int U>
+__device__ __forceinline__ void csr_pair_step(const int32_t *__restrict__ col, const double *__restrict__ val,
+		const double *__restrict__ x, int b, int j0, int j1, int jl, double *p)
+{
+	const int lane = lane_id();
+	i32x2 c[U];
+	f64x2 v[U];
+#pragma unroll
+	for (int u = 0; u < U; u++) {
+		const int j = b + u * 2 * kWave + 2 * lane;
+		load_pair(col, val, j < jl ? j : jl, c[u], v[u]);
+	}
+	double xv[2 * U];
+#pragma unroll
+	for (int u = 0; u < U; u++) {
+		xv[2 * u] = x[c[u].x];
+		xv[2 * u + 1] = x[c[u].y];
+	}
+#pragma unroll
+	for (int u = 0; u < U; u++) {
+		const int j = b + u * 2 * kWave + 2 * lane;
+		if (j >= j0 && j < j1)
+			p[j - j0] = v[u].x * xv[2 * u];
+		if (j + 1 >= j0 && j + 1 < j1)
+			p[j + 1 - j0] = v[u].y * xv[2 * u + 1];
+	}
+}
+
 // One row block [r0, r1) of the binned CSR format (see csr.hip), processed
 // by the calling wavefront. `p` is the wave's LDS buffer of 2 * BNNZ_MAX
 // doubles. A single row longer than BNNZ_MAX is summed by the whole wave;
@@ -164,10 +196,11 @@ __device__ __forceinline__ void block_rows_reduce(const double *p, int ra, int n
 // The block bounds are wave-uniform (scalar registers), so the control below
 // runs on the scalar unit. Lane i loads the bounds of row r0 + i before the
 // stream phase, so the reduction does not wait for a second round of
-// row-pointer loads. The stream phase is branch-free: 4 x 64 entries are
-// requested per step, lanes past the block's end re-read its last entry and
-// store a zero product, so a block of <= 256 nonzeros costs one index round
-// trip and one gather round trip.
+// row-pointer loads. The stream phase has no per-lane branches: 4 x 64
+// entries are requested per step (in the paired layout, a last step with at
+// most 128 entries left requests only those), lanes past the block's end
+// re-read its last entry, so a block of <= 256 nonzeros costs one index
+// round trip and one gather round trip.
 //
 // csr_block_at takes the block's nonzero range [j0, j1) = [rp[r0], rp[r1])
 // already known (a precomputed work-unit record); csr_block loads it.
@@ -205,29 +238,19 @@ __device__ __forceinline__ void csr_block_at(const int32_t *__restrict__ rp, con
 	const int ra = rp[r0 + (lane < nrows ? lane : nrows)] - j0;
 	if (!NARROW) {
 		// stream phase, entry pairs: p[j - j0] for j in [j0, j1) (nz < 2 *
-		// BNNZ_MAX for a multi-row block, so p stays inside the buffer)
+		// BNNZ_MAX for a multi-row block, so p stays inside the buffer).
+		// Steps of 256 entries (both halves) while more than 128 remain,
+		// then one step of 128: a block of about 256 entries that spills
+		// past a 256-entry step costs 4 more vector-memory instructions,
+		// not 8 (the branch is wave-uniform)
 		const int jl = (j1 - 1) & ~1; // the block's last pair (used only when j1 > b >= 0)
-		for (int b = j0 & ~1; b < j1; b += 4 * kWave) {
-			i32x2 c[2];
-			f64x2 v[2];
-#pragma unroll
-			for (int u = 0; u < 2; u++) {
-				const int j = b + u * 2 * kWave + 2 * lane;
-				load_pair(col, val, j < jl ? j : jl, c[u], v[u]);
-			}
-			double xv[4];
-#pragma unroll
-			for (int u = 0; u < 2; u++) {
-				xv[2 * u] = x[c[u].x];
-				xv[2 * u + 1] = x[c[u].y];
-			}
-#pragma unroll
-			for (int u = 0; u < 2; u++) {
-				const int j = b + u * 2 * kWave + 2 * lane;
-				if (j >= j0 && j < j1)
-					p[j - j0] = v[u].x * xv[2 * u];
-				if (j + 1 >= j0 && j + 1 < j1)
-					p[j + 1 - j0] = v[u].y * xv[2 * u + 1];
+		for (int b = j0 & ~1; b < j1;) {
+			if (j1 - b > 2 * kWave) {
+				csr_pair_step<2>(col, val, x, b, j0, j1, jl, p);
+				b += 4 * kWave;
+			} else {
+				csr_pair_step<1>(col, val, x, b, j0, j1, jl, p);
+				b += 2 * kWave;
 			}
 		}
 		wave_lds_sync();
