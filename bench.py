@@ -492,7 +492,7 @@ def main():
                     help="batched launch: every member as this variant (index into the variant list), no selector")
     ap.add_argument("--batch-bnnz", type=int, default=256,
                     help="batched launch: CSR choices with smaller row blocks run with this block size (0 = as chosen)")
-    ap.add_argument("--batch-div-margin", type=float, default=0.1,
+    ap.add_argument("--batch-div-margin", type=float, default=0.25,
                     help="batched launch: a DIV-RF choice is kept only when predicted this much faster than the "
                          "best other variant (DIV members run in a second persistent kernel)")
     ap.add_argument("--queues", type=int, default=256,

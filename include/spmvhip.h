@@ -248,7 +248,8 @@ int spmvhip_batch_set_xcd_ranges(spmvhip_batch *b, int on);
 int spmvhip_batch_set_solo_nnz(spmvhip_batch *b, long nnz);
 int spmvhip_batch_add(spmvhip_batch *b, const spmvhip_format *f, const double *d_x, double *d_y);
 int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph);
-int spmvhip_batch_run(spmvhip_batch *b);          /* asynchronous on the batch's stream */
+int spmvhip_batch_run(spmvhip_batch *b);          /* asynchronous on the device's shared stream (the
+                                                     one the format handles use) */
 int spmvhip_batch_synchronize(spmvhip_batch *b);
 /* ms[i]: GPU time of iteration i of the batched launch (per_matrix = 0) or
  * of one standalone spmv launch per member, back to back (per_matrix = 1).

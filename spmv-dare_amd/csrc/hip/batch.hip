@@ -309,10 +309,10 @@ int spmvhip_batch_create(int tile_units, spmvhip_batch **out)
 		return set_error(SPMVHIP_ENOMEM, "batch handle");
 	if (tile_units > 0)
 		b->tile = tile_units;
-	hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-	if (e != hipSuccess) {
+	const int rc = shared_stream(&b->stream); // the device's one stream (capi.hip)
+	if (rc) {
 		delete b;
-		return set_error(SPMVHIP_EHIP, "stream: %s", hipGetErrorString(e));
+		return rc;
 	}
 	*out = b;
 	return SPMVHIP_OK;
@@ -564,12 +564,24 @@ int spmvhip_batch_finalize(spmvhip_batch *b, int use_graph)
 	}
 	b->finalized = true;
 	if (use_graph) {
-		SPMVHIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-		int rc = batch_launch(b, b->stream);
+		// captured on a stream of its own, so that no other work enqueued
+		// on the shared stream meanwhile can enter the graph; replayed on
+		// the shared stream
+		hipStream_t cap = nullptr;
+		SPMVHIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+		hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+		int rc = SPMVHIP_OK;
 		hipGraph_t graph = nullptr;
-		hipError_t e = hipStreamEndCapture(b->stream, &graph);
-		if (rc)
+		if (e == hipSuccess) {
+			rc = batch_launch(b, cap);
+			e = hipStreamEndCapture(cap, &graph);
+		}
+		(void) hipStreamDestroy(cap);
+		if (rc) {
+			if (graph)
+				(void) hipGraphDestroy(graph);
 			return rc;
+		}
 		if (e != hipSuccess)
 			return set_error(SPMVHIP_EHIP, "graph capture: %s", hipGetErrorString(e));
 		b->graph = graph;
@@ -684,10 +696,7 @@ void spmvhip_batch_destroy(spmvhip_batch *b)
 		(void) hipGraphExecDestroy(b->exec);
 	if (b->graph)
 		(void) hipGraphDestroy(b->graph);
-	hipStream_t s = b->stream;
-	delete b;
-	if (s)
-		(void) hipStreamDestroy(s);
+	delete b; // the stream is the device's shared one: it stays
 }
 
 } // extern "C"

@@ -49,16 +49,19 @@ static int ensure_device()
 	return SPMVHIP_OK;
 }
 
-// One non-blocking stream per device, shared by every format handle on that
-// device (its conversion, and the spmv calls that name no stream). The HIP
-// runtime backs each stream with host resources of its own (a worker thread
-// per queue), so a stream per handle does not scale to the thousands of
-// handles a batched workload holds (two or three per matrix).
+// One non-blocking stream per device, shared by every format handle and
+// batched launch on that device (conversions, the spmv calls that name no
+// stream, batched launches). The HIP runtime backs each stream with host
+// resources of its own (a worker thread per queue), so a stream per handle
+// does not scale to the thousands of handles a batched workload holds (two
+// or three per matrix); and streams land on different hardware queues, on
+// which the same persistent launch measured 2-3 % apart, so every launch of
+// the layer goes through the one queue.
 constexpr int kMaxDevices = 64;
 static std::mutex g_stream_mu;
 static hipStream_t g_streams[kMaxDevices];
 
-static int shared_stream(hipStream_t *out)
+int shared_stream(hipStream_t *out)
 {
 	int dev = 0;
 	SPMVHIP_CHECK(hipGetDevice(&dev));

@@ -93,6 +93,8 @@ int div_convert(spmvhip_format *f, const int32_t *h_row_ptr, const int32_t *h_co
 		const double *h_val);
 int div_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s);
 int div_export_decoded(const spmvhip_format *f, int which, void *buf, size_t *bytes);
+// The current device's stream shared by every handle and batch (capi.hip).
+int shared_stream(hipStream_t *out);
 // The format's own SpMV on stream s (the kernels spmvhip_spmv_device runs).
 int format_spmv(spmvhip_format *f, const double *x, double *y, hipStream_t s);
 

@@ -160,40 +160,37 @@ def batch_variant(i, min_block_nnz=256):
     """The variant a matrix runs as inside a persistent batched launch.
 
     The selector and the autotuner rank variants by standalone launches,
-    where a small matrix gains from small CSR row blocks (more waves in
-    flight). In a batch the other members supply that parallelism, and a
-    small block only adds per-unit overhead (descriptor loads, a reduction)
-    to a wave's sequential stream, so a CSR choice with fewer than
-    `min_block_nnz` nonzeros per block is raised to that block size with the
-    same lanes-per-row setting (0 keeps every block size), and a CSR choice
-    with narrow loads runs with paired loads. Other choices are kept.
+    where a small matrix gains from small CSR row blocks, one entry per lane
+    and load, or a fixed reduction width (more waves in flight, fewer
+    idle lanes). In a batch the other members supply that parallelism, and
+    each of these only adds per-unit work to a wave's sequential stream:
+    every member as one variant, 2,048 matrices, on one stream, measured the
+    default (256-nonzero blocks, paired loads, automatic reduction width)
+    best, narrow loads 2 % and one lane per row 1 % slower. So a CSR choice
+    runs as that default, with row blocks of max(256, min_block_nnz) if its
+    own are smaller (min_block_nnz 0 keeps a choice's block size). Other
+    formats are kept.
     """
     kind, kw = VARIANTS[i]
     if kind != "csr_vec":
         return i
-    want = dict(kw)
-    # one stored entry per lane and load helps a small matrix alone (more
-    # waves in flight); in a batch the paired loads' fewer instructions win
-    # (every member as either, 2,048 matrices: 9.12 against 8.96 ms)
-    want.pop("narrow_loads", None)
-    if min_block_nnz and kw.get("block_nnz", 256) < min_block_nnz:
-        want["block_nnz"] = min_block_nnz
-    if want == kw:
-        return i
+    bnnz = kw.get("block_nnz", 256)
+    want = {"block_nnz": max(bnnz, min_block_nnz) if min_block_nnz else bnnz, "lanes_per_row": 0}
     for j, (k2, kw2) in enumerate(VARIANTS):
         if k2 == kind and kw2 == want:
             return j
     return i
 
 
-def batch_choice(pred, allowed, min_block_nnz=256, div_margin=0.1):
+def batch_choice(pred, allowed, min_block_nnz=256, div_margin=0.25):
     """The variant a matrix runs as inside a persistent batched launch, from
     the selector's predicted slowdowns `pred` (log t_v / t_best, one row).
 
     DIV-RF members run in a second persistent kernel after the one that
-    takes every other member, so a DIV member adds that kernel's ramp and
-    tail to the launch; a DIV choice is kept only when it is predicted at
-    least `div_margin` faster than the best other allowed variant (else that
+    takes every other member, so DIV members add that kernel's launch, ramp
+    and tail (on 2,048 small matrices, two DIV members cost the launch about
+    1 %); a DIV choice is kept only when it is predicted at least
+    `div_margin` faster than the best other allowed variant (else that
     variant runs). The choice then goes through batch_variant."""
     pred = np.where(np.asarray(allowed, dtype=bool), np.asarray(pred, dtype=np.float64), np.inf)
     best = int(np.argmin(pred))

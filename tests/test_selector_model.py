@@ -106,20 +106,21 @@ def test_batch_choice_div_margin():
 
 
 def test_batch_variant_mapping():
-    """Inside a batch: CSR blocks below the batch's block size are raised to
-    it with the same lanes per row, narrow loads become paired loads, other
-    choices are kept; 0 keeps every block size."""
+    """Inside a batch every CSR choice runs as the default CSR unit (paired
+    loads, automatic reduction width) with blocks of at least the batch's
+    block size; min_block_nnz 0 keeps the block size; other formats are
+    kept."""
     V = tuning.VARIANTS
     idx = lambda k, kw: V.index((k, kw))  # noqa: E731
-    assert tuning.batch_variant(idx("csr_vec", {"block_nnz": 64, "lanes_per_row": 1}), 256) == \
-        idx("csr_vec", {"block_nnz": 256, "lanes_per_row": 1})
-    assert tuning.batch_variant(idx("csr_vec", {"block_nnz": 128, "lanes_per_row": 4}), 256) == \
-        idx("csr_vec", {"block_nnz": 256, "lanes_per_row": 4})
-    narrow = idx("csr_vec", {"block_nnz": 256, "lanes_per_row": 0, "narrow_loads": 1})
-    assert tuning.batch_variant(narrow, 256) == idx("csr_vec", {"block_nnz": 256, "lanes_per_row": 0})
-    assert tuning.batch_variant(narrow, 0) == idx("csr_vec", {"block_nnz": 256, "lanes_per_row": 0})
-    small = idx("csr_vec", {"block_nnz": 64, "lanes_per_row": 0})
-    assert tuning.batch_variant(small, 0) == small
+    d256 = idx("csr_vec", {"block_nnz": 256, "lanes_per_row": 0})
+    for kw in ({"block_nnz": 64, "lanes_per_row": 1}, {"block_nnz": 128, "lanes_per_row": 4},
+               {"block_nnz": 256, "lanes_per_row": 0, "narrow_loads": 1},
+               {"block_nnz": 256, "lanes_per_row": 4, "xcd_remap": 1}, {"block_nnz": 256, "lanes_per_row": 0}):
+        assert tuning.batch_variant(idx("csr_vec", kw), 256) == d256
+    assert tuning.batch_variant(idx("csr_vec", {"block_nnz": 512, "lanes_per_row": 1}), 256) == \
+        idx("csr_vec", {"block_nnz": 512, "lanes_per_row": 0})
+    assert tuning.batch_variant(idx("csr_vec", {"block_nnz": 64, "lanes_per_row": 1}), 0) == \
+        idx("csr_vec", {"block_nnz": 64, "lanes_per_row": 0})
     for k, kw in V:
         if k != "csr_vec":
             assert tuning.batch_variant(V.index((k, kw)), 256) == V.index((k, kw))
