@@ -7,9 +7,8 @@ import subprocess
 import sys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-sys.path.insert(0, ROOT)
 
-import bench  # noqa: E402  (bench.py runs nothing at import)
+import bench  # noqa: E402  (the repository root is on the path from conftest; bench.py runs nothing at import)
 
 
 def test_profiled_traffic_rows_present():
